@@ -1,0 +1,250 @@
+// C entry points over the CPU restatement (hf_oracle.hpp), for the tests'
+// ctypes checker and bench.py's CPU-baseline leg. TEST INFRASTRUCTURE ONLY:
+// the product library never links or loads this.
+//
+// The functions mirror the product C ABI (include/hydroflux_amd.h) so one
+// test body can drive both: AoS (h, hu, hv) triples in row-major order, and
+// a parallel AoS array of tangents for the Dual kind.
+#include <omp.h>
+
+#include <cstdint>
+#include <cstring>
+
+#include "hf_oracle.hpp"
+
+using namespace hfo;
+
+namespace {
+struct Handle {
+  int dual = 0;
+  Sim<double> f;
+  Sim<Dual> d;
+};
+
+Bc make_bc(int kind, double p) {
+  Bc b;
+  b.kind = kind;
+  b.param = p;
+  return b;
+}
+}  // namespace
+
+extern "C" {
+
+void* hfo_create(int64_t nr, int64_t nc, double dx, double dy, int dual) {
+  Handle* h = new Handle();
+  h->dual = dual;
+  if (dual)
+    h->d.init(nr, nc, dx, dy);
+  else
+    h->f.init(nr, nc, dx, dy);
+  return h;
+}
+
+void hfo_destroy(void* p) { delete static_cast<Handle*>(p); }
+
+void hfo_set_pow_mode(int mode) { pow_mode() = mode; }
+void hfo_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+
+// bed, manning: n cells each; tangents nullable (Dual only).
+void hfo_set_mesh(void* p, const double* bed, const double* bed_tan, const double* manning,
+                  const double* manning_tan) {
+  Handle* h = static_cast<Handle*>(p);
+  if (!h->dual) {
+    size_t n = h->f.mesh.bed.size();
+    for (size_t k = 0; k < n; ++k) {
+      h->f.mesh.bed[k] = bed[k];
+      h->f.mesh.manning[k] = manning[k];
+    }
+  } else {
+    size_t n = h->d.mesh.bed.size();
+    for (size_t k = 0; k < n; ++k) {
+      h->d.mesh.bed[k] = Dual{bed[k], bed_tan ? bed_tan[k] : 0.0};
+      h->d.mesh.manning[k] = Dual{manning[k], manning_tan ? manning_tan[k] : 0.0};
+    }
+  }
+}
+
+void hfo_set_state(void* p, const double* aos, const double* aos_tan) {
+  Handle* h = static_cast<Handle*>(p);
+  if (!h->dual) {
+    size_t n = h->f.st.size();
+    for (size_t k = 0; k < n; ++k) h->f.st[k] = {aos[3 * k], aos[3 * k + 1], aos[3 * k + 2]};
+  } else {
+    size_t n = h->d.st.size();
+    for (size_t k = 0; k < n; ++k) {
+      double t0 = aos_tan ? aos_tan[3 * k] : 0.0, t1 = aos_tan ? aos_tan[3 * k + 1] : 0.0,
+             t2 = aos_tan ? aos_tan[3 * k + 2] : 0.0;
+      h->d.st[k] = {Dual{aos[3 * k], t0}, Dual{aos[3 * k + 1], t1}, Dual{aos[3 * k + 2], t2}};
+    }
+  }
+}
+
+void hfo_get_state(void* p, double* aos, double* aos_tan) {
+  Handle* h = static_cast<Handle*>(p);
+  if (!h->dual) {
+    size_t n = h->f.st.size();
+    for (size_t k = 0; k < n; ++k) {
+      aos[3 * k] = h->f.st[k].h;
+      aos[3 * k + 1] = h->f.st[k].hu;
+      aos[3 * k + 2] = h->f.st[k].hv;
+    }
+  } else {
+    size_t n = h->d.st.size();
+    for (size_t k = 0; k < n; ++k) {
+      aos[3 * k] = h->d.st[k].h.val;
+      aos[3 * k + 1] = h->d.st[k].hu.val;
+      aos[3 * k + 2] = h->d.st[k].hv.val;
+      if (aos_tan) {
+        aos_tan[3 * k] = h->d.st[k].h.dval;
+        aos_tan[3 * k + 1] = h->d.st[k].hu.dval;
+        aos_tan[3 * k + 2] = h->d.st[k].hv.dval;
+      }
+    }
+  }
+}
+
+// kinds/params ordered N, S, W, E (boundary.rs:131-140).
+void hfo_set_boundaries(void* p, const int* kinds, const double* params) {
+  Handle* h = static_cast<Handle*>(p);
+  Bcs b;
+  b.north = make_bc(kinds[0], params[0]);
+  b.south = make_bc(kinds[1], params[1]);
+  b.west = make_bc(kinds[2], params[2]);
+  b.east = make_bc(kinds[3], params[3]);
+  if (h->dual)
+    h->d.bcs = b;
+  else
+    h->f.bcs = b;
+}
+
+void hfo_max_wave_speeds(void* p, double* sx, double* sy) {
+  Handle* h = static_cast<Handle*>(p);
+  if (h->dual)
+    max_wave_speeds(h->d.st, sx, sy);
+  else
+    max_wave_speeds(h->f.st, sx, sy);
+}
+
+double hfo_cfl_time_step(void* p, double cfl, int with_bcs) {
+  Handle* h = static_cast<Handle*>(p);
+  if (h->dual)
+    return with_bcs ? cfl_time_step_with_bcs(h->d.st, h->d.mesh, h->d.bcs, cfl)
+                    : cfl_time_step(h->d.st, h->d.mesh, cfl);
+  return with_bcs ? cfl_time_step_with_bcs(h->f.st, h->f.mesh, h->f.bcs, cfl)
+                  : cfl_time_step(h->f.st, h->f.mesh, cfl);
+}
+
+void hfo_forward_euler_step(void* p, double dt) {
+  Handle* h = static_cast<Handle*>(p);
+  if (h->dual)
+    forward_euler_step_with(h->d.st, h->d.mesh, h->d.bcs, dt, h->d.ws);
+  else
+    forward_euler_step_with(h->f.st, h->f.mesh, h->f.bcs, dt, h->f.ws);
+}
+
+void hfo_ssprk2_step(void* p, double dt) {
+  Handle* h = static_cast<Handle*>(p);
+  if (h->dual)
+    ssprk2_step_with(h->d.st, h->d.mesh, h->d.bcs, dt, h->d.ws);
+  else
+    ssprk2_step_with(h->f.st, h->f.mesh, h->f.bcs, dt, h->f.ws);
+}
+
+void hfo_manning_friction_step(void* p, double dt, double dry_tol) {
+  Handle* h = static_cast<Handle*>(p);
+  if (h->dual)
+    manning_friction_step(h->d.st, h->d.mesh, dt, dry_tol);
+  else
+    manning_friction_step(h->f.st, h->f.mesh, dt, dry_tol);
+}
+
+void hfo_apply_rain(void* p, double rate, double dt) {
+  Handle* h = static_cast<Handle*>(p);
+  if (h->dual)
+    apply_rain(h->d.st, rate, dt);
+  else
+    apply_rain(h->f.st, rate, dt);
+}
+
+static std::vector<PointSource> make_srcs(const int64_t* rows, const int64_t* cols,
+                                          const double* q, int64_t k) {
+  std::vector<PointSource> v;
+  for (int64_t s = 0; s < k; ++s) v.push_back(PointSource{rows[s], cols[s], q[s]});
+  return v;
+}
+
+void hfo_apply_point_sources(void* p, const int64_t* rows, const int64_t* cols, const double* q,
+                             int64_t k, double dt) {
+  Handle* h = static_cast<Handle*>(p);
+  std::vector<PointSource> v = make_srcs(rows, cols, q, k);
+  if (h->dual)
+    apply_point_sources(h->d.st, h->d.mesh.nc, v, dt, h->d.mesh.dx, h->d.mesh.dy);
+  else
+    apply_point_sources(h->f.st, h->f.mesh.nc, v, dt, h->f.mesh.dx, h->f.mesh.dy);
+}
+
+// Driver configuration (sim.rs:49-77 plus the forcing extension).
+int hfo_configure(void* p, double cfl, int integrator, int cfl_with_bcs, double max_dt,
+                  int friction, double friction_dry_tol, int64_t max_steps, double rain_rate,
+                  const int64_t* rows, const int64_t* cols, const double* q, int64_t k) {
+  Handle* h = static_cast<Handle*>(p);
+  if (!(cfl > 0.0 && cfl <= 1.0)) return ST_INVALID_CONFIG;  // sim.rs:141-146
+  if (!(max_dt > 0.0)) return ST_INVALID_CONFIG;             // sim.rs:147-152
+  Config c;
+  c.cfl = cfl;
+  c.integrator = integrator;
+  c.cfl_with_bcs = cfl_with_bcs;
+  c.max_dt = max_dt;
+  c.friction = friction;
+  c.friction_dry_tol = friction_dry_tol;
+  c.max_steps = max_steps;
+  c.rain_rate = rain_rate;
+  std::vector<PointSource> v = make_srcs(rows, cols, q, k);
+  if (h->dual) {
+    h->d.cfg = c;
+    h->d.srcs = v;
+  } else {
+    h->f.cfg = c;
+    h->f.srcs = v;
+  }
+  return ST_OK;
+}
+
+int hfo_step(void* p, double dt_cap, double* dt_taken) {
+  Handle* h = static_cast<Handle*>(p);
+  return h->dual ? h->d.step(dt_cap, dt_taken) : h->f.step(dt_cap, dt_taken);
+}
+
+int hfo_run_until(void* p, double t_target) {
+  Handle* h = static_cast<Handle*>(p);
+  return h->dual ? h->d.run_until(t_target) : h->f.run_until(t_target);
+}
+
+int hfo_run_steps(void* p, int64_t n) {
+  Handle* h = static_cast<Handle*>(p);
+  return h->dual ? h->d.run_steps(n) : h->f.run_steps(n);
+}
+
+void hfo_time(void* p, double* t, int64_t* steps) {
+  Handle* h = static_cast<Handle*>(p);
+  *t = h->dual ? h->d.t : h->f.t;
+  *steps = h->dual ? h->d.steps : h->f.steps;
+}
+
+void hfo_set_time(void* p, double t, int64_t steps) {
+  Handle* h = static_cast<Handle*>(p);
+  if (h->dual) {
+    h->d.t = t;
+    h->d.steps = steps;
+  } else {
+    h->f.t = t;
+    h->f.steps = steps;
+  }
+}
+
+// Correctly rounded powers shared with the kernels, exposed for tests.
+double hfo_pow_4_3(double x) { return hfm::pow_4_3(x); }
+double hfo_pow_1_3m(double x) { return hfm::pow_1_3m(x); }
+double hfo_libm_pow(double x, double n) { return std::pow(x, n); }
+}
