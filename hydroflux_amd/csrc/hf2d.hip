@@ -1,0 +1,942 @@
+// MI355X (gfx950) implementation of the C ABI in include/hydroflux_amd.h.
+//
+// Device layout: SoA float64 planes (h, hu, hv) x 2 state buffers, bed z,
+// Manning n, plus tangent planes for the Dual kind. One SSP-RK2 step is
+// three launches on the handle's stream:
+//   stage kernel (first)  : U^n            -> U^(1)   (dt from the device clock)
+//   stage kernel (final)  : U^(1), U^n     -> U^{n+1} (+combine, friction, rain,
+//                                                       point sources, CFL maxima)
+//   finalize (1 thread)   : t += dt, steps += 1, flip the CFL slot
+// so the host never reads dt back; run_until polls the clock every chunk.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/hydroflux_amd.h"
+#include "hf2d_tile.h"
+
+using hf::Ctrl;
+using hf::dual;
+using hf::Fld;
+using hf::FldW;
+using hf::PointSrc;
+using hf::StageArgs;
+
+namespace {
+
+// ------------------------------------------------------------ tile shapes
+template <class T> struct Shape;
+template <> struct Shape<double> {
+  static constexpr int TW = 32, TH = 16, NT = 256;
+};
+template <> struct Shape<dual> {
+  static constexpr int TW = 16, TH = 16, NT = 256;
+};
+
+__device__ inline void block_max_to(double sx, double sy, double (*red)[16], unsigned long long* dst) {
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    sx = __builtin_fmax(sx, __shfl_xor(sx, off));
+    sy = __builtin_fmax(sy, __shfl_xor(sy, off));
+  }
+  const int wave = tid >> 6, lane = tid & 63;
+  if (lane == 0) {
+    red[0][wave] = sx;
+    red[1][wave] = sy;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    const int nw = (blockDim.x + 63) >> 6;
+    for (int w = 1; w < nw; ++w) {
+      sx = __builtin_fmax(sx, red[0][w]);
+      sy = __builtin_fmax(sy, red[1][w]);
+    }
+    atomicMax(dst + 0, hfm::as_u64(sx));
+    atomicMax(dst + 1, hfm::as_u64(sy));
+  }
+}
+
+template <class T, int MODE>
+__global__ __launch_bounds__(Shape<T>::NT) void stage_kernel(StageArgs<T> a) {
+  constexpr int TW = Shape<T>::TW, TH = Shape<T>::TH, NT = Shape<T>::NT;
+  using St = hf::Stage<T, TW, TH>;
+  __shared__ typename St::S s;
+  double dt;
+  if (!hf::stage_prologue(a, blockIdx.x == 0 && threadIdx.x == 0, &dt)) return;
+  const int64_t ty = blockIdx.x / a.tiles_x, tx = blockIdx.x % a.tiles_x;
+  const int64_t gi0 = ty * TH, gj0 = tx * TW;
+  const int tid = threadIdx.x;
+  St::phase_load(s, a, gi0, gj0, tid, NT);
+  __syncthreads();
+  St::phase_slopes(s, a, gi0, gj0, tid, NT);
+  __syncthreads();
+  St::phase_faces(s, a, gi0, gj0, tid, NT);
+  __syncthreads();
+  St::phase_alpha(s, a, gi0, gj0, tid, NT, dt);
+  __syncthreads();
+  double wsx = 0.0, wsy = 0.0;
+  St::template phase_update<MODE>(s, a, gi0, gj0, tid, NT, dt, &wsx, &wsy);
+  if (MODE == hf::MODE_FINAL_RK2 && a.wave_speed) {
+    Ctrl* c = a.ctrl;
+    block_max_to(wsx, wsy, s.red, &c->ws[c->parity ^ 1][0]);
+  }
+}
+
+// Forward-Euler tail and the standalone source steps: elementwise over
+// U_in -> U_out with the optional forcing and CFL maxima.
+template <class T>
+__global__ __launch_bounds__(256) void tail_kernel(StageArgs<T> a, int slot_mode) {
+  __shared__ double red[2][16];
+  double dt;
+  if (!hf::stage_prologue(a, blockIdx.x == 0 && threadIdx.x == 0, &dt)) return;
+  using St = hf::Stage<T, 32, 16>;
+  double wsx = 0.0, wsy = 0.0;
+  const int64_t n = a.nr * a.nc;
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < n;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t gi = idx / a.nc, gj = idx % a.nc;
+    T h = a.ih.ld(idx), hu = a.ihu.ld(idx), hv = a.ihv.ld(idx);
+    St::forcing(a, gi, gj, dt, &h, &hu, &hv);
+    a.oh.st(idx, h);
+    a.ohu.st(idx, hu);
+    a.ohv.st(idx, hv);
+    if (a.wave_speed) {
+      T zc = a.z.ld(idx);
+      T zn4[4];
+      zn4[hf::SIDE_N] = (a.nr >= 2 && gi == 0) ? a.z.ld(idx + a.nc) : zc;
+      zn4[hf::SIDE_S] = (a.nr >= 2 && gi == a.nr - 1) ? a.z.ld(idx - a.nc) : zc;
+      zn4[hf::SIDE_W] = (a.nc >= 2 && gj == 0) ? a.z.ld(idx + 1) : zc;
+      zn4[hf::SIDE_E] = (a.nc >= 2 && gj == a.nc - 1) ? a.z.ld(idx - 1) : zc;
+      T man = a.man.ld(idx);
+      St::cell_wave_speed(a, gi, gj, h, hu, hv, zc, zn4, &man, &wsx, &wsy);
+    }
+  }
+  if (a.wave_speed) {
+    Ctrl* c = a.ctrl;
+    block_max_to(wsx, wsy, red, &c->ws[slot_mode ? c->parity : (c->parity ^ 1)][0]);
+  }
+}
+
+// apply_point_sources, sequential in list order (source.rs:52-69).
+template <class T> __global__ void point_source_kernel(StageArgs<T> a, double dt) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const T Z = hf::cst<T>(0.0);
+  const double cell_area = a.dx * a.dy;
+  for (int k = 0; k < a.n_src; ++k) {
+    const int64_t idx = a.srcs[k].row * a.nc + a.srcs[k].col;
+    const double dh = a.srcs[k].q * dt / cell_area;
+    T h = hf::rmax(a.ih.ld(idx) + dh, Z);
+    T hu = a.ihu.ld(idx), hv = a.ihv.ld(idx);
+    if (hf::val(h) == 0.0) {
+      hu = Z;
+      hv = Z;
+    }
+    a.oh.st(idx, h);
+    a.ohu.st(idx, hu);
+    a.ohv.st(idx, hv);
+  }
+}
+
+__global__ void finalize_kernel(Ctrl* c) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  if (c->status != 0 || c->done != 0) return;
+  c->t = c->t + c->dt;  // sim.rs:225 (plain sum)
+  c->steps = c->steps + 1;
+  c->budget = c->budget - 1;
+  c->parity = c->parity ^ 1;
+}
+
+__global__ void zero_slot_kernel(Ctrl* c) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    c->ws[c->parity][0] = 0ULL;
+    c->ws[c->parity][1] = 0ULL;
+  }
+}
+
+// Plain streaming copy (dst = src), the calibration kernel for HBM counters
+// and the achievable-bandwidth reference.
+__global__ __launch_bounds__(256) void copy_kernel(const double2* __restrict__ src, double2* __restrict__ dst,
+                                                   int64_t n2) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n2; i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
+__global__ void math_probe_kernel(const double* x, const double* y, int64_t n, double* out) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double xi = x[i];
+  out[4 * i + 0] = __builtin_sqrt(xi);
+  out[4 * i + 1] = xi / y[i];
+  out[4 * i + 2] = hfm::pow_4_3(xi);
+  out[4 * i + 3] = hfm::pow_1_3m(xi);
+}
+
+}  // namespace
+
+// ============================================================== handle
+struct hf2d_sim {
+  int64_t nr = 0, nc = 0;
+  double dx = 1.0, dy = 1.0;
+  int kind = HF2D_F64;
+  hipStream_t stream = nullptr;
+  double* st[2][3] = {{nullptr}};   // state buffers
+  double* tg[2][3] = {{nullptr}};   // tangents (Dual)
+  double *z = nullptr, *zt = nullptr, *man = nullptr, *mant = nullptr;
+  bool has_zt = false;
+  int cur = 0;  // index of the buffer holding the current state
+  Ctrl* ctrl = nullptr;
+  PointSrc* srcs = nullptr;
+  int n_src = 0, cap_src = 0;
+  int bc_kind[4] = {HF2D_BC_WALL, HF2D_BC_WALL, HF2D_BC_WALL, HF2D_BC_WALL};
+  double bc_param[4] = {0, 0, 0, 0};
+  hf2d_config cfg{};
+  bool ws_valid = false;  // ctrl->ws[parity] holds the current state's maxima
+  bool ws_with_bcs = false;
+  std::string err;
+  // kernel timing
+  bool timing = false;
+  std::vector<hipEvent_t> ev;  // pairs: [s1 start, s1 end, s2 start, s2 end] per step
+  size_t ev_used = 0;
+  double timed_ms[2] = {0, 0};
+  int64_t timed_count = 0;
+};
+
+namespace {
+
+int fail(hf2d_sim* s, int code, const std::string& msg) {
+  if (s) s->err = msg;
+  return code;
+}
+
+#define HIPCHK(sim, expr)                                                                 \
+  do {                                                                                    \
+    hipError_t e_ = (expr);                                                               \
+    if (e_ != hipSuccess)                                                                 \
+      return fail(sim, HF2D_HIP_ERROR, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+hf2d_config default_config() {
+  hf2d_config c{};
+  c.cfl = 0.4;
+  c.integrator = HF2D_SSPRK2;
+  c.cfl_with_bcs = 1;
+  c.max_dt = 60.0;
+  c.friction = 1;
+  c.friction_dry_tol = hf::H_DRY;
+  c.max_steps = 10000000;
+  c.rain_rate = 0.0;
+  return c;
+}
+
+template <class T> Fld<T> fld(const double* v, const double* d);
+template <> Fld<double> fld<double>(const double* v, const double*) { return Fld<double>{v}; }
+template <> Fld<dual> fld<dual>(const double* v, const double* d) { return Fld<dual>{v, d}; }
+template <class T> FldW<T> fldw(double* v, double* d);
+template <> FldW<double> fldw<double>(double* v, double*) { return FldW<double>{v}; }
+template <> FldW<dual> fldw<dual>(double* v, double* d) { return FldW<dual>{v, d}; }
+
+// Arguments for a launch reading state buffer `in`, writing buffer `out`,
+// with U^n in buffer `un`.
+template <class T> StageArgs<T> make_args(hf2d_sim* s, int in, int out, int un) {
+  StageArgs<T> a{};
+  a.nr = s->nr;
+  a.nc = s->nc;
+  a.dx = s->dx;
+  a.dy = s->dy;
+  a.ih = fld<T>(s->st[in][0], s->tg[in][0]);
+  a.ihu = fld<T>(s->st[in][1], s->tg[in][1]);
+  a.ihv = fld<T>(s->st[in][2], s->tg[in][2]);
+  a.oh = fldw<T>(s->st[out][0], s->tg[out][0]);
+  a.ohu = fldw<T>(s->st[out][1], s->tg[out][1]);
+  a.ohv = fldw<T>(s->st[out][2], s->tg[out][2]);
+  a.nh = fld<T>(s->st[un][0], s->tg[un][0]);
+  a.nhu = fld<T>(s->st[un][1], s->tg[un][1]);
+  a.nhv = fld<T>(s->st[un][2], s->tg[un][2]);
+  a.z = fld<T>(s->z, s->has_zt ? s->zt : nullptr);
+  a.man = fld<T>(s->man, s->mant);
+  for (int k = 0; k < 4; ++k) {
+    a.bc_kind[k] = s->bc_kind[k];
+    a.bc_param[k] = s->bc_param[k];
+  }
+  a.ctrl = s->ctrl;
+  a.cfl = s->cfg.cfl;
+  a.max_dt = s->cfg.max_dt;
+  a.cfl_with_bcs = s->cfg.cfl_with_bcs;
+  a.srcs = s->srcs;
+  constexpr int TW = Shape<T>::TW, TH = Shape<T>::TH;
+  a.tiles_x = (int)((s->nc + TW - 1) / TW);
+  a.tiles_y = (int)((s->nr + TH - 1) / TH);
+  return a;
+}
+
+template <class T> int launch_stage(hf2d_sim* s, StageArgs<T> a, int mode, int timing_slot) {
+  const unsigned grid = (unsigned)((int64_t)a.tiles_x * a.tiles_y);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (s->timing && timing_slot >= 0) {
+    if (s->ev_used + 2 > s->ev.size()) {
+      for (int k = 0; k < 64; ++k) {
+        hipEvent_t e;
+        HIPCHK(s, hipEventCreate(&e));
+        s->ev.push_back(e);
+      }
+    }
+    e0 = s->ev[s->ev_used];
+    e1 = s->ev[s->ev_used + 1];
+    s->ev_used += 2;
+    HIPCHK(s, hipEventRecord(e0, s->stream));
+  }
+  if (mode == hf::MODE_STAGE)
+    hipLaunchKernelGGL((stage_kernel<T, hf::MODE_STAGE>), dim3(grid), dim3(Shape<T>::NT), 0, s->stream, a);
+  else
+    hipLaunchKernelGGL((stage_kernel<T, hf::MODE_FINAL_RK2>), dim3(grid), dim3(Shape<T>::NT), 0, s->stream,
+                       a);
+  HIPCHK(s, hipGetLastError());
+  if (e1) {
+    HIPCHK(s, hipEventRecord(e1, s->stream));
+    (void)timing_slot;
+  }
+  return HF2D_OK;
+}
+
+// Harvest finished timing event pairs (order: stage-1 pair, final pair).
+int harvest_timing(hf2d_sim* s) {
+  if (!s->timing || s->ev_used == 0) return HF2D_OK;
+  HIPCHK(s, hipStreamSynchronize(s->stream));
+  for (size_t k = 0; k + 3 < s->ev_used; k += 4) {
+    float ms0 = 0, ms1 = 0;
+    HIPCHK(s, hipEventElapsedTime(&ms0, s->ev[k], s->ev[k + 1]));
+    HIPCHK(s, hipEventElapsedTime(&ms1, s->ev[k + 2], s->ev[k + 3]));
+    s->timed_ms[0] += ms0;
+    s->timed_ms[1] += ms1;
+    s->timed_count += 1;
+  }
+  s->ev_used = 0;
+  return HF2D_OK;
+}
+
+unsigned tail_grid(hf2d_sim* s) {
+  int64_t n = s->nr * s->nc;
+  int64_t g = (n + 255) / 256;
+  if (g > 65536) g = 65536;
+  if (g < 1) g = 1;
+  return (unsigned)g;
+}
+
+// Make ctrl->ws[parity] hold the CFL maxima of the current state.
+template <class T> int refresh_wave_speeds(hf2d_sim* s, bool with_bcs) {
+  if (s->ws_valid && s->ws_with_bcs == with_bcs) return HF2D_OK;
+  hipLaunchKernelGGL(zero_slot_kernel, dim3(1), dim3(1), 0, s->stream, s->ctrl);
+  HIPCHK(s, hipGetLastError());
+  StageArgs<T> a = make_args<T>(s, s->cur, s->cur, s->cur);
+  a.dt_mode = 0;
+  a.dt_host = 0.0;
+  a.friction = 0;
+  a.rain_rate = 0.0;
+  a.n_src = 0;
+  a.wave_speed = 1;
+  a.cfl_with_bcs = with_bcs ? 1 : 0;
+  hipLaunchKernelGGL((tail_kernel<T>), dim3(tail_grid(s)), dim3(256), 0, s->stream, a, 1);
+  HIPCHK(s, hipGetLastError());
+  s->ws_valid = true;
+  s->ws_with_bcs = with_bcs;
+  return HF2D_OK;
+}
+
+template <class T> int do_max_wave_speeds(hf2d_sim* s, bool with_bcs, double* sx, double* sy) {
+  int rc = refresh_wave_speeds<T>(s, with_bcs);
+  if (rc) return rc;
+  Ctrl c;
+  HIPCHK(s, hipMemcpyAsync(&c, s->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(s, hipStreamSynchronize(s->stream));
+  *sx = hfm::as_f64(c.ws[c.parity][0]);
+  *sy = hfm::as_f64(c.ws[c.parity][1]);
+  return HF2D_OK;
+}
+
+// One step with caller dt: forward Euler or SSP-RK2 without forcing.
+template <class T> int host_dt_step(hf2d_sim* s, double dt, bool rk2) {
+  const int cur = s->cur, oth = 1 - s->cur;
+  StageArgs<T> a = make_args<T>(s, cur, oth, cur);
+  a.dt_mode = 0;
+  a.dt_host = dt;
+  int rc = launch_stage<T>(s, a, hf::MODE_STAGE, -1);
+  if (rc) return rc;
+  if (rk2) {
+    StageArgs<T> b = make_args<T>(s, oth, cur, cur);
+    b.dt_mode = 0;
+    b.dt_host = dt;
+    b.friction = 0;
+    b.rain_rate = 0.0;
+    b.n_src = 0;
+    b.wave_speed = 0;
+    rc = launch_stage<T>(s, b, hf::MODE_FINAL_RK2, -1);
+    if (rc) return rc;
+  } else {
+    s->cur = oth;
+  }
+  s->ws_valid = false;
+  return HF2D_OK;
+}
+
+// Enqueue one device-clocked step of the configured driver.
+template <class T> int enqueue_step(hf2d_sim* s) {
+  const int cur = s->cur, oth = 1 - s->cur;
+  StageArgs<T> a = make_args<T>(s, cur, oth, cur);
+  a.dt_mode = 1;
+  a.first_stage = 1;
+  StageArgs<T> b = make_args<T>(s, oth, cur, cur);
+  b.dt_mode = 1;
+  b.first_stage = 0;
+  b.friction = s->cfg.friction;
+  b.friction_dry_tol = s->cfg.friction_dry_tol;
+  b.rain_rate = s->cfg.rain_rate;
+  b.n_src = s->n_src;
+  b.wave_speed = 1;
+  if (s->cfg.integrator == HF2D_SSPRK2) {
+    int rc = launch_stage<T>(s, a, hf::MODE_STAGE, 0);
+    if (rc) return rc;
+    rc = launch_stage<T>(s, b, hf::MODE_FINAL_RK2, 1);
+    if (rc) return rc;
+  } else {
+    // Forward Euler: stage into the other buffer, then the elementwise
+    // tail copies it back with forcing and the CFL maxima.
+    int rc = launch_stage<T>(s, a, hf::MODE_STAGE, 0);
+    if (rc) return rc;
+    StageArgs<T> t = b;
+    t.ih = fld<T>(s->st[oth][0], s->tg[oth][0]);
+    t.ihu = fld<T>(s->st[oth][1], s->tg[oth][1]);
+    t.ihv = fld<T>(s->st[oth][2], s->tg[oth][2]);
+    t.oh = fldw<T>(s->st[cur][0], s->tg[cur][0]);
+    t.ohu = fldw<T>(s->st[cur][1], s->tg[cur][1]);
+    t.ohv = fldw<T>(s->st[cur][2], s->tg[cur][2]);
+    if (s->timing) {
+      // keep the event pairing (stage, final) for the tail
+      if (s->ev_used + 2 > s->ev.size()) {
+        for (int k = 0; k < 64; ++k) {
+          hipEvent_t e;
+          HIPCHK(s, hipEventCreate(&e));
+          s->ev.push_back(e);
+        }
+      }
+      HIPCHK(s, hipEventRecord(s->ev[s->ev_used], s->stream));
+    }
+    hipLaunchKernelGGL((tail_kernel<T>), dim3(tail_grid(s)), dim3(256), 0, s->stream, t, 0);
+    HIPCHK(s, hipGetLastError());
+    if (s->timing) {
+      HIPCHK(s, hipEventRecord(s->ev[s->ev_used + 1], s->stream));
+      s->ev_used += 2;
+    }
+  }
+  hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(1), 0, s->stream, s->ctrl);
+  HIPCHK(s, hipGetLastError());
+  return HF2D_OK;
+}
+
+template <class T> int prepare_driver(hf2d_sim* s) {
+  return refresh_wave_speeds<T>(s, s->cfg.cfl_with_bcs != 0);
+}
+
+int read_ctrl(hf2d_sim* s, Ctrl* c) {
+  HIPCHK(s, hipMemcpyAsync(c, s->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(s, hipStreamSynchronize(s->stream));
+  return HF2D_OK;
+}
+
+int write_ctrl(hf2d_sim* s, const Ctrl* c) {
+  HIPCHK(s, hipMemcpyAsync(s->ctrl, c, sizeof(Ctrl), hipMemcpyHostToDevice, s->stream));
+  HIPCHK(s, hipStreamSynchronize(s->stream));
+  return HF2D_OK;
+}
+
+int ctrl_error(hf2d_sim* s, const Ctrl& c, double t_target) {
+  char buf[256];
+  if (c.status == 3) {
+    snprintf(buf, sizeof buf, "degenerate time step dt = %g at t = %g (step %lld)", c.dt, c.t, c.steps);
+    return fail(s, HF2D_DEGENERATE_DT, buf);
+  }
+  if (c.status == 4) {
+    snprintf(buf, sizeof buf, "step budget exhausted: %lld steps, t = %g of %g s", c.steps, c.t, t_target);
+    return fail(s, HF2D_STEP_BUDGET_EXHAUSTED, buf);
+  }
+  return HF2D_OK;
+}
+
+template <class T> int run_driver(hf2d_sim* s, bool use_target, double t_target, double dt_cap,
+                                  int64_t n_steps, bool poll) {
+  int rc = prepare_driver<T>(s);
+  if (rc) return rc;
+  Ctrl c;
+  rc = read_ctrl(s, &c);
+  if (rc) return rc;
+  c.status = 0;
+  c.done = 0;
+  c.use_target = use_target ? 1 : 0;
+  c.t_target = t_target;
+  c.dt_cap = dt_cap;
+  c.budget = use_target ? s->cfg.max_steps : (long long)n_steps;
+  rc = write_ctrl(s, &c);
+  if (rc) return rc;
+  s->ws_valid = false;  // the slots now follow the driver
+  if (!use_target) {
+    for (int64_t k = 0; k < n_steps; ++k) {
+      rc = enqueue_step<T>(s);
+      if (rc) return rc;
+    }
+    if (!poll) return HF2D_OK;
+    rc = read_ctrl(s, &c);
+    if (rc) return rc;
+    rc = harvest_timing(s);
+    if (rc) return rc;
+    s->ws_valid = (c.status == 0);
+    s->ws_with_bcs = s->cfg.cfl_with_bcs != 0;
+    return ctrl_error(s, c, t_target);
+  }
+  // run_until: enqueue in chunks, poll the device clock between chunks.
+  int64_t chunk = 16;
+  for (;;) {
+    for (int64_t k = 0; k < chunk; ++k) {
+      rc = enqueue_step<T>(s);
+      if (rc) return rc;
+    }
+    rc = read_ctrl(s, &c);
+    if (rc) return rc;
+    rc = harvest_timing(s);
+    if (rc) return rc;
+    if (c.status != 0 || c.done != 0) break;
+    if (chunk < 256) chunk *= 2;
+  }
+  s->ws_valid = (c.status == 0);
+  s->ws_with_bcs = s->cfg.cfl_with_bcs != 0;
+  return ctrl_error(s, c, t_target);
+}
+
+bool valid(const hf2d_sim* s) { return s != nullptr; }
+
+int check_bcs_geometry(hf2d_sim* s, const int32_t kinds[4]) {
+  for (int k = 0; k < 4; ++k) {
+    if (kinds[k] < 0 || kinds[k] > 3) return fail(s, HF2D_INVALID_CONFIG, "unknown boundary kind");
+    bool physical = kinds[k] == HF2D_BC_DISCHARGE || kinds[k] == HF2D_BC_DEPTH;
+    bool xf = (k == hf::SIDE_W || k == hf::SIDE_E);
+    // boundary.rs:349-364: linear bed extrapolation needs two cells across.
+    if (physical && (xf ? s->nc < 2 : s->nr < 2))
+      return fail(s, HF2D_INVALID_CONFIG, "physical boundary needs at least 2 cells across the face");
+  }
+  return HF2D_OK;
+}
+
+}  // namespace
+
+namespace {
+template <class T> int forcing_only(hf2d_sim* s, double dt, int friction, double dry_tol, double rain) {
+  StageArgs<T> a = make_args<T>(s, s->cur, s->cur, s->cur);
+  a.dt_mode = 0;
+  a.dt_host = dt;
+  a.friction = friction;
+  a.friction_dry_tol = dry_tol;
+  a.rain_rate = rain;
+  a.n_src = 0;
+  a.wave_speed = 0;
+  hipLaunchKernelGGL((tail_kernel<T>), dim3(tail_grid(s)), dim3(256), 0, s->stream, a, 0);
+  HIPCHK(s, hipGetLastError());
+  s->ws_valid = false;
+  return HF2D_OK;
+}
+
+int upload_sources(hf2d_sim* s, const hf2d_point_source* src, int64_t k) {
+  for (int64_t i = 0; i < k; ++i)
+    if (src[i].row < 0 || src[i].row >= s->nr || src[i].col < 0 || src[i].col >= s->nc)
+      return fail(s, HF2D_SHAPE_MISMATCH, "point source outside the grid");
+  if (k > s->cap_src) {
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    if (s->srcs) HIPCHK(s, hipFree(s->srcs));
+    s->srcs = nullptr;
+    HIPCHK(s, hipMalloc(&s->srcs, (size_t)k * sizeof(PointSrc)));
+    s->cap_src = (int)k;
+  }
+  std::vector<PointSrc> v((size_t)k);
+  for (int64_t i = 0; i < k; ++i) v[(size_t)i] = PointSrc{src[i].row, src[i].col, src[i].q_mass};
+  if (k > 0) {
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    HIPCHK(s, hipMemcpy(s->srcs, v.data(), (size_t)k * sizeof(PointSrc), hipMemcpyHostToDevice));
+  }
+  s->n_src = (int)k;
+  return HF2D_OK;
+}
+}  // namespace
+
+// ============================================================== C ABI
+extern "C" {
+
+int hf2d_create(int64_t n_rows, int64_t n_cols, double dx, double dy, int32_t scalar_kind, hf2d_sim** out) {
+  if (!out) return HF2D_INVALID_ARGUMENT;
+  *out = nullptr;
+  if (scalar_kind != HF2D_F64 && scalar_kind != HF2D_DUAL) return HF2D_INVALID_ARGUMENT;
+  if (!(dx > 0.0) || !(dy > 0.0) || n_rows <= 0 || n_cols <= 0) return HF2D_INVALID_CONFIG;
+  hf2d_sim* s = new hf2d_sim();
+  s->nr = n_rows;
+  s->nc = n_cols;
+  s->dx = dx;
+  s->dy = dy;
+  s->kind = scalar_kind;
+  s->cfg = default_config();
+  const size_t bytes = (size_t)(n_rows * n_cols) * sizeof(double);
+  auto cleanup = [&](int code) {
+    hf2d_destroy(s);
+    return code;
+  };
+  if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) return cleanup(HF2D_HIP_ERROR);
+  for (int b = 0; b < 2; ++b)
+    for (int f = 0; f < 3; ++f) {
+      if (hipMalloc(&s->st[b][f], bytes) != hipSuccess) return cleanup(HF2D_HIP_ERROR);
+      if (hipMemset(s->st[b][f], 0, bytes) != hipSuccess) return cleanup(HF2D_HIP_ERROR);
+      if (scalar_kind == HF2D_DUAL) {
+        if (hipMalloc(&s->tg[b][f], bytes) != hipSuccess) return cleanup(HF2D_HIP_ERROR);
+        if (hipMemset(s->tg[b][f], 0, bytes) != hipSuccess) return cleanup(HF2D_HIP_ERROR);
+      }
+    }
+  if (hipMalloc(&s->z, bytes) != hipSuccess || hipMemset(s->z, 0, bytes) != hipSuccess) return cleanup(HF2D_HIP_ERROR);
+  if (hipMalloc(&s->man, bytes) != hipSuccess || hipMemset(s->man, 0, bytes) != hipSuccess)
+    return cleanup(HF2D_HIP_ERROR);
+  if (scalar_kind == HF2D_DUAL) {
+    if (hipMalloc(&s->zt, bytes) != hipSuccess || hipMemset(s->zt, 0, bytes) != hipSuccess)
+      return cleanup(HF2D_HIP_ERROR);
+    if (hipMalloc(&s->mant, bytes) != hipSuccess || hipMemset(s->mant, 0, bytes) != hipSuccess)
+      return cleanup(HF2D_HIP_ERROR);
+  }
+  if (hipMalloc(&s->ctrl, sizeof(Ctrl)) != hipSuccess) return cleanup(HF2D_HIP_ERROR);
+  Ctrl c{};
+  c.t_target = 0.0;
+  c.dt_cap = __builtin_inf();
+  if (hipMemcpy(s->ctrl, &c, sizeof(Ctrl), hipMemcpyHostToDevice) != hipSuccess) return cleanup(HF2D_HIP_ERROR);
+  *out = s;
+  return HF2D_OK;
+}
+
+int hf2d_destroy(hf2d_sim* s) {
+  if (!s) return HF2D_OK;
+  if (s->stream) (void)hipStreamSynchronize(s->stream);
+  for (int b = 0; b < 2; ++b)
+    for (int f = 0; f < 3; ++f) {
+      if (s->st[b][f]) (void)hipFree(s->st[b][f]);
+      if (s->tg[b][f]) (void)hipFree(s->tg[b][f]);
+    }
+  if (s->z) (void)hipFree(s->z);
+  if (s->zt) (void)hipFree(s->zt);
+  if (s->man) (void)hipFree(s->man);
+  if (s->mant) (void)hipFree(s->mant);
+  if (s->ctrl) (void)hipFree(s->ctrl);
+  if (s->srcs) (void)hipFree(s->srcs);
+  for (hipEvent_t e : s->ev) (void)hipEventDestroy(e);
+  if (s->stream) (void)hipStreamDestroy(s->stream);
+  delete s;
+  return HF2D_OK;
+}
+
+const char* hf2d_last_error(const hf2d_sim* s) { return s ? s->err.c_str() : "null handle"; }
+
+int hf2d_upload_mesh(hf2d_sim* s, const double* bed, const double* bed_tangent, const double* manning,
+                     const double* manning_tangent) {
+  if (!valid(s) || !bed || !manning) return fail(s, HF2D_INVALID_ARGUMENT, "null mesh array");
+  const size_t n = (size_t)(s->nr * s->nc);
+  for (size_t k = 0; k < n; ++k)
+    if (!(manning[k] >= 0.0)) return fail(s, HF2D_INVALID_CONFIG, "Manning n must be non-negative at every cell");
+  const size_t bytes = n * sizeof(double);
+  HIPCHK(s, hipMemcpy(s->z, bed, bytes, hipMemcpyHostToDevice));
+  HIPCHK(s, hipMemcpy(s->man, manning, bytes, hipMemcpyHostToDevice));
+  if (s->kind == HF2D_DUAL) {
+    s->has_zt = bed_tangent != nullptr;
+    if (bed_tangent)
+      HIPCHK(s, hipMemcpy(s->zt, bed_tangent, bytes, hipMemcpyHostToDevice));
+    else
+      HIPCHK(s, hipMemset(s->zt, 0, bytes));
+    if (manning_tangent)
+      HIPCHK(s, hipMemcpy(s->mant, manning_tangent, bytes, hipMemcpyHostToDevice));
+    else
+      HIPCHK(s, hipMemset(s->mant, 0, bytes));
+  }
+  s->ws_valid = false;
+  return HF2D_OK;
+}
+
+int hf2d_upload_state(hf2d_sim* s, const double* aos, const double* tan_aos) {
+  if (!valid(s) || !aos) return fail(s, HF2D_INVALID_ARGUMENT, "null state array");
+  HIPCHK(s, hipStreamSynchronize(s->stream));
+  const size_t n = (size_t)(s->nr * s->nc);
+  std::vector<double> plane(n);
+  for (int f = 0; f < 3; ++f) {
+    for (size_t k = 0; k < n; ++k) plane[k] = aos[3 * k + f];
+    HIPCHK(s, hipMemcpy(s->st[s->cur][f], plane.data(), n * sizeof(double), hipMemcpyHostToDevice));
+    if (s->kind == HF2D_DUAL) {
+      for (size_t k = 0; k < n; ++k) plane[k] = tan_aos ? tan_aos[3 * k + f] : 0.0;
+      HIPCHK(s, hipMemcpy(s->tg[s->cur][f], plane.data(), n * sizeof(double), hipMemcpyHostToDevice));
+    }
+  }
+  s->ws_valid = false;
+  return HF2D_OK;
+}
+
+int hf2d_download_state(hf2d_sim* s, double* aos, double* tan_aos) {
+  if (!valid(s) || !aos) return fail(s, HF2D_INVALID_ARGUMENT, "null state array");
+  HIPCHK(s, hipStreamSynchronize(s->stream));
+  const size_t n = (size_t)(s->nr * s->nc);
+  std::vector<double> plane(n);
+  for (int f = 0; f < 3; ++f) {
+    HIPCHK(s, hipMemcpy(plane.data(), s->st[s->cur][f], n * sizeof(double), hipMemcpyDeviceToHost));
+    for (size_t k = 0; k < n; ++k) aos[3 * k + f] = plane[k];
+    if (s->kind == HF2D_DUAL && tan_aos) {
+      HIPCHK(s, hipMemcpy(plane.data(), s->tg[s->cur][f], n * sizeof(double), hipMemcpyDeviceToHost));
+      for (size_t k = 0; k < n; ++k) tan_aos[3 * k + f] = plane[k];
+    }
+  }
+  return HF2D_OK;
+}
+
+int hf2d_set_boundaries(hf2d_sim* s, const int32_t kinds[4], const double params[4]) {
+  if (!valid(s) || !kinds) return fail(s, HF2D_INVALID_ARGUMENT, "null boundary array");
+  int rc = check_bcs_geometry(s, kinds);
+  if (rc) return rc;
+  for (int k = 0; k < 4; ++k) {
+    s->bc_kind[k] = kinds[k];
+    s->bc_param[k] = params ? params[k] : 0.0;
+  }
+  s->ws_valid = false;
+  return HF2D_OK;
+}
+
+int hf2d_max_wave_speeds(hf2d_sim* s, double* sx, double* sy) {
+  if (!valid(s) || !sx || !sy) return fail(s, HF2D_INVALID_ARGUMENT, "null argument");
+  return s->kind == HF2D_DUAL ? do_max_wave_speeds<dual>(s, false, sx, sy)
+                              : do_max_wave_speeds<double>(s, false, sx, sy);
+}
+
+int hf2d_cfl_time_step(hf2d_sim* s, double cfl, int32_t with_bcs, double* dt) {
+  if (!valid(s) || !dt) return fail(s, HF2D_INVALID_ARGUMENT, "null argument");
+  double sx, sy;
+  int rc = s->kind == HF2D_DUAL ? do_max_wave_speeds<dual>(s, with_bcs != 0, &sx, &sy)
+                                : do_max_wave_speeds<double>(s, with_bcs != 0, &sx, &sy);
+  if (rc) return rc;
+  const double denom = sx / s->dx + sy / s->dy;  // update.rs:150-155
+  *dt = denom > 0.0 ? cfl / denom : __builtin_inf();
+  return HF2D_OK;
+}
+
+int hf2d_forward_euler_step(hf2d_sim* s, double dt) {
+  if (!valid(s)) return HF2D_INVALID_ARGUMENT;
+  return s->kind == HF2D_DUAL ? host_dt_step<dual>(s, dt, false) : host_dt_step<double>(s, dt, false);
+}
+
+int hf2d_ssprk2_step(hf2d_sim* s, double dt) {
+  if (!valid(s)) return HF2D_INVALID_ARGUMENT;
+  return s->kind == HF2D_DUAL ? host_dt_step<dual>(s, dt, true) : host_dt_step<double>(s, dt, true);
+}
+
+
+int hf2d_manning_friction_step(hf2d_sim* s, double dt, double dry_tol) {
+  if (!valid(s)) return HF2D_INVALID_ARGUMENT;
+  return s->kind == HF2D_DUAL ? forcing_only<dual>(s, dt, 1, dry_tol, 0.0)
+                              : forcing_only<double>(s, dt, 1, dry_tol, 0.0);
+}
+
+int hf2d_apply_rain(hf2d_sim* s, double rate, double dt) {
+  if (!valid(s)) return HF2D_INVALID_ARGUMENT;
+  if (rate == 0.0) return HF2D_OK;  // source.rs:89-91
+  return s->kind == HF2D_DUAL ? forcing_only<dual>(s, dt, 0, 0.0, rate)
+                              : forcing_only<double>(s, dt, 0, 0.0, rate);
+}
+
+int hf2d_apply_point_sources(hf2d_sim* s, const hf2d_point_source* src, int64_t k, double dt) {
+  if (!valid(s) || (k > 0 && !src)) return fail(s, HF2D_INVALID_ARGUMENT, "null sources");
+  if (k == 0) return HF2D_OK;
+  // Temporarily swap in this list; the configured driver list is restored.
+  std::vector<hf2d_point_source> keep;
+  int rc = HF2D_OK;
+  PointSrc* saved = s->srcs;
+  int saved_n = s->n_src, saved_cap = s->cap_src;
+  s->srcs = nullptr;
+  s->n_src = 0;
+  s->cap_src = 0;
+  rc = upload_sources(s, src, k);
+  if (rc == HF2D_OK) {
+    if (s->kind == HF2D_DUAL) {
+      StageArgs<dual> a = make_args<dual>(s, s->cur, s->cur, s->cur);
+      a.n_src = s->n_src;
+      hipLaunchKernelGGL((point_source_kernel<dual>), dim3(1), dim3(64), 0, s->stream, a, dt);
+    } else {
+      StageArgs<double> a = make_args<double>(s, s->cur, s->cur, s->cur);
+      a.n_src = s->n_src;
+      hipLaunchKernelGGL((point_source_kernel<double>), dim3(1), dim3(64), 0, s->stream, a, dt);
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) rc = fail(s, HF2D_HIP_ERROR, hipGetErrorString(e));
+    hipStreamSynchronize(s->stream);
+  }
+  if (s->srcs) (void)hipFree(s->srcs);
+  s->srcs = saved;
+  s->n_src = saved_n;
+  s->cap_src = saved_cap;
+  s->ws_valid = false;
+  return rc;
+}
+
+int hf2d_configure(hf2d_sim* s, const hf2d_config* cfg, const hf2d_point_source* src, int64_t k) {
+  if (!valid(s) || !cfg) return fail(s, HF2D_INVALID_ARGUMENT, "null config");
+  if (!(cfg->cfl > 0.0 && cfg->cfl <= 1.0)) {
+    char buf[128];
+    snprintf(buf, sizeof buf, "invalid configuration: cfl must be in (0, 1], got %g", cfg->cfl);
+    return fail(s, HF2D_INVALID_CONFIG, buf);
+  }
+  if (!(cfg->max_dt > 0.0)) {
+    char buf[128];
+    snprintf(buf, sizeof buf, "invalid configuration: max_dt must be positive, got %g", cfg->max_dt);
+    return fail(s, HF2D_INVALID_CONFIG, buf);
+  }
+  if (cfg->integrator != HF2D_SSPRK2 && cfg->integrator != HF2D_FORWARD_EULER)
+    return fail(s, HF2D_INVALID_CONFIG, "invalid configuration: unknown integrator");
+  if (k > 0 && !src) return fail(s, HF2D_INVALID_ARGUMENT, "null sources");
+  int rc = upload_sources(s, src, k);
+  if (rc) return rc;
+  s->cfg = *cfg;
+  s->ws_valid = false;
+  return HF2D_OK;
+}
+
+int hf2d_step(hf2d_sim* s, double dt_cap, double* dt_taken) {
+  if (!valid(s)) return HF2D_INVALID_ARGUMENT;
+  int rc = s->kind == HF2D_DUAL ? run_driver<dual>(s, false, 0.0, dt_cap, 1, true)
+                                : run_driver<double>(s, false, 0.0, dt_cap, 1, true);
+  if (dt_taken) {
+    Ctrl c;
+    if (read_ctrl(s, &c) == HF2D_OK) *dt_taken = c.dt;
+  }
+  return rc;
+}
+
+int hf2d_run_until(hf2d_sim* s, double t_target) {
+  if (!valid(s)) return HF2D_INVALID_ARGUMENT;
+  return s->kind == HF2D_DUAL ? run_driver<dual>(s, true, t_target, 0.0, 0, true)
+                              : run_driver<double>(s, true, t_target, 0.0, 0, true);
+}
+
+int hf2d_run_steps(hf2d_sim* s, int64_t n) {
+  if (!valid(s)) return HF2D_INVALID_ARGUMENT;
+  if (n <= 0) return HF2D_OK;
+  return s->kind == HF2D_DUAL ? run_driver<dual>(s, false, 0.0, __builtin_inf(), n, true)
+                              : run_driver<double>(s, false, 0.0, __builtin_inf(), n, true);
+}
+
+int hf2d_enqueue_steps(hf2d_sim* s, int64_t n) {
+  if (!valid(s)) return HF2D_INVALID_ARGUMENT;
+  if (n <= 0) return HF2D_OK;
+  return s->kind == HF2D_DUAL ? run_driver<dual>(s, false, 0.0, __builtin_inf(), n, false)
+                              : run_driver<double>(s, false, 0.0, __builtin_inf(), n, false);
+}
+
+int hf2d_synchronize(hf2d_sim* s) {
+  if (!valid(s)) return HF2D_INVALID_ARGUMENT;
+  HIPCHK(s, hipStreamSynchronize(s->stream));
+  Ctrl c;
+  int rc = read_ctrl(s, &c);
+  if (rc) return rc;
+  rc = harvest_timing(s);
+  if (rc) return rc;
+  s->ws_valid = (c.status == 0);
+  s->ws_with_bcs = s->cfg.cfl_with_bcs != 0;
+  return ctrl_error(s, c, c.t_target);
+}
+
+int hf2d_get_time(hf2d_sim* s, double* t, int64_t* steps) {
+  if (!valid(s)) return HF2D_INVALID_ARGUMENT;
+  Ctrl c;
+  int rc = read_ctrl(s, &c);
+  if (rc) return rc;
+  if (t) *t = c.t;
+  if (steps) *steps = c.steps;
+  return HF2D_OK;
+}
+
+int hf2d_set_time(hf2d_sim* s, double t, int64_t steps) {
+  if (!valid(s)) return HF2D_INVALID_ARGUMENT;
+  Ctrl c;
+  int rc = read_ctrl(s, &c);
+  if (rc) return rc;
+  c.t = t;
+  c.steps = steps;
+  return write_ctrl(s, &c);
+}
+
+int hf2d_set_kernel_timing(hf2d_sim* s, int32_t enabled) {
+  if (!valid(s)) return HF2D_INVALID_ARGUMENT;
+  int rc = harvest_timing(s);
+  if (rc) return rc;
+  s->timing = enabled != 0;
+  s->timed_ms[0] = s->timed_ms[1] = 0.0;
+  s->timed_count = 0;
+  s->ev_used = 0;
+  return HF2D_OK;
+}
+
+int hf2d_kernel_timing(hf2d_sim* s, double stage_ms[2], int64_t* count) {
+  if (!valid(s)) return HF2D_INVALID_ARGUMENT;
+  int rc = harvest_timing(s);
+  if (rc) return rc;
+  if (stage_ms) {
+    stage_ms[0] = s->timed_ms[0];
+    stage_ms[1] = s->timed_ms[1];
+  }
+  if (count) *count = s->timed_count;
+  return HF2D_OK;
+}
+
+int hf2d_copy_probe(int64_t n_bytes, int32_t reps, double* ms_avg) {
+  if (n_bytes <= 0 || reps <= 0 || !ms_avg) return HF2D_INVALID_ARGUMENT;
+  const int64_t n2 = n_bytes / 16;
+  double2 *a = nullptr, *b = nullptr;
+  if (hipMalloc(&a, (size_t)n2 * 16) != hipSuccess) return HF2D_HIP_ERROR;
+  if (hipMalloc(&b, (size_t)n2 * 16) != hipSuccess) {
+    (void)hipFree(a);
+    return HF2D_HIP_ERROR;
+  }
+  (void)hipMemset(a, 0, (size_t)n2 * 16);
+  hipEvent_t e0, e1;
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  int64_t grid = (n2 + 255) / 256;
+  if (grid > 8192) grid = 8192;
+  hipLaunchKernelGGL(copy_kernel, dim3((unsigned)grid), dim3(256), 0, 0, a, b, n2);  // warm
+  (void)hipEventRecord(e0, 0);
+  for (int r = 0; r < reps; ++r) hipLaunchKernelGGL(copy_kernel, dim3((unsigned)grid), dim3(256), 0, 0, a, b, n2);
+  (void)hipEventRecord(e1, 0);
+  hipError_t e = hipEventSynchronize(e1);
+  float ms = 0.f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+  *ms_avg = (double)ms / reps;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(a);
+  (void)hipFree(b);
+  return e == hipSuccess ? HF2D_OK : HF2D_HIP_ERROR;
+}
+
+int hf2d_math_probe(const double* x, const double* y, int64_t n, double* out) {
+  if (!x || !y || !out || n <= 0) return HF2D_INVALID_ARGUMENT;
+  double *dx_ = nullptr, *dy_ = nullptr, *dout = nullptr;
+  const size_t b = (size_t)n * sizeof(double);
+  if (hipMalloc(&dx_, b) != hipSuccess) return HF2D_HIP_ERROR;
+  if (hipMalloc(&dy_, b) != hipSuccess) return HF2D_HIP_ERROR;
+  if (hipMalloc(&dout, 4 * b) != hipSuccess) return HF2D_HIP_ERROR;
+  hipMemcpy(dx_, x, b, hipMemcpyHostToDevice);
+  hipMemcpy(dy_, y, b, hipMemcpyHostToDevice);
+  hipLaunchKernelGGL(math_probe_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, 0, dx_, dy_, n, dout);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpy(out, dout, 4 * b, hipMemcpyDeviceToHost);
+  hipFree(dx_);
+  hipFree(dy_);
+  hipFree(dout);
+  return e == hipSuccess ? HF2D_OK : HF2D_HIP_ERROR;
+}
+
+}  // extern "C"

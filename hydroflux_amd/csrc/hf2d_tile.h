@@ -1,0 +1,879 @@
+// One SSP-RK2 / forward-Euler stage of hydroflux solver-2d on one output tile.
+//
+// The body is written as barrier-separated phases over a tile-local LDS
+// struct so that the identical code runs (a) on gfx950 inside the
+// __global__ wrappers of hf2d.hip, with __syncthreads() between phases, and
+// (b) on the CPU in the tests' emulator, which runs each phase for every
+// thread index in turn (a valid schedule of a barrier-separated program).
+//
+// Arithmetic follows the reference expression by expression
+// (solver-2d/src/update.rs:215-1211, riemann.rs:114-204, boundary.rs:193-367,
+// source.rs:52-165); the build uses -ffp-contract=off so no a*b+c is fused.
+//
+// Dependency radius (the reason for the halo sizes): a cell's update needs
+// the four faces around it, each scaled by the upstream cell's alpha
+// (update.rs:992-1053). Alpha of a ring-1 neighbour needs that neighbour's
+// four raw faces, whose MUSCL slopes read primitives one cell further out,
+// and a slope is zeroed if any cell within +-2 along its axis is dry
+// (update.rs:433-444). Hence: state (h, z, u, v) on a 3-cell halo, dry flags
+// on a 4-cell halo, slopes on a 2-cell halo (along their axis; 1 across),
+// faces and alpha on a 1-cell ring.
+//
+// Local coordinates: tile origin (gi0, gj0); li = gi - gi0 in [0, TH),
+// lj = gj - gj0 in [0, TW). x-face fj sits between local cells fj-1 and fj;
+// y-face fi between local rows fi-1 and fi.
+#pragma once
+#include <stdint.h>
+
+#include "hf_scalar.h"
+
+namespace hf {
+
+// solver-2d/src/lib.rs:43,63,72; riemann.rs:74; boundary.rs:68,76
+constexpr double GRAVITY = 9.81;
+constexpr double H_DRY = 1.0e-6;
+constexpr double H_VEL = 1.0e-5;
+constexpr double DRY_TOL = 1.0e-12;
+constexpr double DRY_INFLOW_SLOPE_THRESHOLD = 5.0e-4;
+constexpr double DRY_INFLOW_H_MAX = 10.0;
+
+enum { BC_TRANSMISSIVE = 0, BC_WALL = 1, BC_DISCHARGE = 2, BC_DEPTH = 3 };
+enum { SIDE_N = 0, SIDE_S = 1, SIDE_W = 2, SIDE_E = 3 };
+enum { MODE_STAGE = 0, MODE_FINAL_RK2 = 1 };
+
+#ifndef HF_BOUNDS_CHECK
+#define HF_BOUNDS_CHECK(cond) ((void)0)
+#endif
+
+// ------------------------------------------------------------ device fields
+template <class T> struct Fld;
+template <> struct Fld<double> {
+  const double* v;
+  HF_HD double ld(int64_t k) const { return v[k]; }
+};
+template <> struct Fld<dual> {
+  const double* v;
+  const double* d;  // may be null: tangent 0 (constant bed)
+  HF_HD dual ld(int64_t k) const { return dual{v[k], d ? d[k] : 0.0}; }
+};
+template <class T> struct FldW;
+template <> struct FldW<double> {
+  double* v;
+  HF_HD void st(int64_t k, double x) const { v[k] = x; }
+};
+template <> struct FldW<dual> {
+  double* v;
+  double* d;
+  HF_HD void st(int64_t k, dual x) const {
+    v[k] = x.v;
+    d[k] = x.d;
+  }
+};
+
+// Device-resident driver state: the clock, the dt of the step in flight and
+// two ping-pong slots of CFL wave-speed maxima (bit patterns of
+// non-negative doubles, so unsigned max is the float max).
+struct Ctrl {
+  double t;
+  double dt;
+  double t_target;
+  double dt_cap;
+  unsigned long long ws[2][2];
+  long long steps;
+  long long budget;
+  int status;  // 0 ok, 3 degenerate dt, 4 step budget exhausted
+  int done;    // run_until reached its target
+  int parity;  // ws[parity] holds the maxima of the current state
+  int use_target;
+};
+
+struct PointSrc {  // source.rs:36-44
+  int64_t row, col;
+  double q;
+};
+
+template <class T> struct StageArgs {
+  int64_t nr, nc;
+  double dx, dy;
+  Fld<T> ih, ihu, ihv;  // stencil input
+  FldW<T> oh, ohu, ohv;  // output
+  Fld<T> nh, nhu, nhv;  // U^n (final SSP-RK2 stage)
+  Fld<T> z, man;
+  int bc_kind[4];  // N, S, W, E
+  double bc_param[4];
+  int dt_mode;  // 0: dt_host; 1: from the device clock
+  int first_stage;
+  double dt_host;
+  Ctrl* ctrl;
+  double cfl, max_dt;
+  int cfl_with_bcs;
+  int friction;
+  double friction_dry_tol;
+  double rain_rate;
+  int n_src;
+  const PointSrc* srcs;
+  int wave_speed;  // reduce the output's wave speeds into ctrl->ws[parity ^ 1]
+  int tiles_x, tiles_y;
+};
+
+// ------------------------------------------------------------ small pieces
+// update.rs:362-370
+template <class T> HF_HD T minmod(T a, T b) {
+  if (val(a) * val(b) <= 0.0) return cst<T>(0.0);
+  if (__builtin_fabs(val(a)) < __builtin_fabs(val(b))) return a;
+  return b;
+}
+
+// update.rs:650-656
+template <class T> HF_HD T interior_z_face(bool l_dry, T z_l, bool r_dry, T z_r) {
+  if (l_dry != r_dry) return rmax(z_l, z_r);
+  return (z_l + z_r) * 0.5;
+}
+
+// boundary.rs:335-367. zc: bed of the boundary cell, zn: bed of the next
+// inner cell along the face normal.
+template <class T> HF_HD T ghost_bed(int kind, T zc, T zn) {
+  if (kind == BC_TRANSMISSIVE || kind == BC_WALL) return zc;
+  return zc * 2.0 - zn;
+}
+
+// boundary.rs:193-219 + 274-333. `d` is dx for West/East, dy for
+// North/South; `have_next` says the mesh has two cells across the normal.
+template <class T>
+HF_HD void ghost_state(int kind, double param, int side, T h, T hu, T hv, T zc, T zn, bool have_next,
+                       double d, T man, T* gh, T* ghu, T* ghv) {
+  const bool xf = (side == SIDE_W || side == SIDE_E);
+  *gh = h;
+  *ghu = hu;
+  *ghv = hv;
+  if (kind == BC_WALL) {
+    if (xf)
+      *ghu = -hu;
+    else
+      *ghv = -hv;
+  } else if (kind == BC_DISCHARGE) {
+    if (xf)
+      *ghu = cst<T>(param);
+    else
+      *ghv = cst<T>(param);
+    if (val(h) < H_DRY) {
+      // bed_slope_x/y (geometry.rs:131-146), magnitude only
+      double slope = have_next ? __builtin_fabs(val((zc - zn) / d)) : 0.0;
+      if (slope > DRY_INFLOW_SLOPE_THRESHOLD) {
+        T hn = powf_gen(man * (__builtin_fabs(param) / __builtin_sqrt(slope)), 3.0 / 5.0);
+        *gh = rmin(hn, cst<T>(DRY_INFLOW_H_MAX));
+      }
+    }
+  } else if (kind == BC_DEPTH) {
+    *gh = cst<T>(param);
+  }
+}
+
+// riemann.rs:114-204
+template <class T>
+HF_HD void hllc_normal_flux(T h_l, T qn_l, T qt_l, T h_r, T qn_r, T qt_r, T* fm, T* fn, T* ft) {
+  const bool lw = val(h_l) > DRY_TOL;
+  const bool rw = val(h_r) > DRY_TOL;
+  const T Z = cst<T>(0.0);
+  if (!lw && !rw) {
+    *fm = Z;
+    *fn = Z;
+    *ft = Z;
+    return;
+  }
+  T cl = lw ? rsqrt_(h_l * GRAVITY) : Z;
+  T cr = rw ? rsqrt_(h_r * GRAVITY) : Z;
+  T un_l = lw ? qn_l / h_l : Z;
+  T un_r = rw ? qn_r / h_r : Z;
+  T ut_l = lw ? qt_l / h_l : Z;
+  T ut_r = rw ? qt_r / h_r : Z;
+  T sl, sr;
+  if (lw && rw) {
+    sl = rmin(un_l - cl, un_r - cr);
+    sr = rmax(un_l + cl, un_r + cr);
+  } else if (rw) {
+    sl = un_r - cr * 2.0;
+    sr = un_r + cr;
+  } else {
+    sl = un_l - cl;
+    sr = un_l + cl * 2.0;
+  }
+  if (val(sl) >= 0.0) {
+    if (lw) {
+      *fm = qn_l;
+      *fn = qn_l * un_l + h_l * h_l * (0.5 * GRAVITY);
+      *ft = qn_l * ut_l;
+    } else {
+      *fm = Z;
+      *fn = Z;
+      *ft = Z;
+    }
+    return;
+  }
+  if (val(sr) <= 0.0) {
+    if (rw) {
+      *fm = qn_r;
+      *fn = qn_r * un_r + h_r * h_r * (0.5 * GRAVITY);
+      *ft = qn_r * ut_r;
+    } else {
+      *fm = Z;
+      *fn = Z;
+      *ft = Z;
+    }
+    return;
+  }
+  T denom = h_r * (un_r - sr) - h_l * (un_l - sl);
+  T s_star = (__builtin_fabs(val(denom)) > 1.0e-12)
+                 ? (h_r * (un_r - sr) * sl - h_l * (un_l - sl) * sr) / denom
+                 : (sl + sr) * 0.5;
+  if (val(s_star) >= 0.0) {
+    T fm_l = Z, fn_l = Z, ft_l = Z;
+    if (lw) {
+      fm_l = qn_l;
+      fn_l = qn_l * un_l + h_l * h_l * (0.5 * GRAVITY);
+      ft_l = qn_l * ut_l;
+    }
+    T h_star_l = h_l * (sl - un_l) / (sl - s_star);
+    *fm = fm_l + (h_star_l - h_l) * sl;
+    *fn = fn_l + (h_star_l * s_star - qn_l) * sl;
+    *ft = ft_l + (h_star_l * ut_l - qt_l) * sl;
+  } else {
+    T fm_r = Z, fn_r = Z, ft_r = Z;
+    if (rw) {
+      fm_r = qn_r;
+      fn_r = qn_r * un_r + h_r * h_r * (0.5 * GRAVITY);
+      ft_r = qn_r * ut_r;
+    }
+    T h_star_r = h_r * (sr - un_r) / (sr - s_star);
+    *fm = fm_r + (h_star_r - h_r) * sr;
+    *fn = fn_r + (h_star_r * s_star - qn_r) * sr;
+    *ft = ft_r + (h_star_r * ut_r - qt_r) * sr;
+  }
+}
+
+// update.rs:256-353 with z_left = z_right = z_face. Outputs the shared
+// mass and tangential fluxes and the two normal-momentum fluxes
+// (minus: consumed by the left/top cell, plus: by the right/bottom cell).
+template <class T>
+HF_HD void well_balanced_face(T lh, T lqn, T lqt, T rh, T rqn, T rqt, T zf, T* mass, T* nm_minus,
+                              T* nm_plus, T* tang) {
+  const T Z = cst<T>(0.0);
+  T z_max = rmax(zf, zf);
+  T hsl = rmax(lh + zf - z_max, Z);
+  T hsr = rmax(rh + zf - z_max, Z);
+  T ul = Z, vl = Z, ur = Z, vr = Z;
+  if (val(lh) > H_VEL) {
+    ul = lqn / lh;
+    vl = lqt / lh;
+  }
+  if (val(rh) > H_VEL) {
+    ur = rqn / rh;
+    vr = rqt / rh;
+  }
+  T fm, fn, ft;
+  hllc_normal_flux(hsl, hsl * ul, hsl * vl, hsr, hsr * ur, hsr * vr, &fm, &fn, &ft);
+  T corr_l = (lh * lh - hsl * hsl) * (0.5 * GRAVITY);
+  T corr_r = (rh * rh - hsr * hsr) * (0.5 * GRAVITY);
+  *mass = fm;
+  *nm_minus = fn + corr_l;
+  *nm_plus = fn + corr_r;
+  *tang = ft;
+}
+
+// ------------------------------------------------------------ tile LDS
+template <class T, int TW, int TH> struct TileSmem {
+  static constexpr int W3 = TW + 6, H3 = TH + 6, N3 = W3 * H3;
+  static constexpr int W4 = TW + 8, H4 = TH + 8, N4 = W4 * H4;
+  static constexpr int WSX = TW + 4, HSX = TH + 2, NSX = WSX * HSX;
+  static constexpr int WSY = TW + 2, HSY = TH + 4, NSY = WSY * HSY;
+  static constexpr int WFX = TW + 3, HFX = TH + 2, NFX = WFX * HFX;
+  static constexpr int WFY = TW + 2, HFY = TH + 3, NFY = WFY * HFY;
+  static constexpr int WA = TW + 2, HA = TH + 2, NA = WA * HA;
+
+  T h[N3], z[N3], u[N3], v[N3];
+  T sxe[NSX], sxu[NSX], sxv[NSX];
+  T sye[NSY], syu[NSY], syv[NSY];
+  T fxm[NFX], fxa[NFX], fxb[NFX], fxt[NFX];
+  T fym[NFY], fya[NFY], fyb[NFY], fyt[NFY];
+  T alpha[NA];
+  unsigned char dry[N4];
+  double red[2][16];
+
+  static HF_HD int i3(int li, int lj) {
+    HF_BOUNDS_CHECK(li >= -3 && li < TH + 3 && lj >= -3 && lj < TW + 3);
+    return (li + 3) * W3 + (lj + 3);
+  }
+  static HF_HD int i4(int li, int lj) {
+    HF_BOUNDS_CHECK(li >= -4 && li < TH + 4 && lj >= -4 && lj < TW + 4);
+    return (li + 4) * W4 + (lj + 4);
+  }
+  static HF_HD int isx(int li, int lj) {
+    HF_BOUNDS_CHECK(li >= -1 && li < TH + 1 && lj >= -2 && lj < TW + 2);
+    return (li + 1) * WSX + (lj + 2);
+  }
+  static HF_HD int isy(int li, int lj) {
+    HF_BOUNDS_CHECK(li >= -2 && li < TH + 2 && lj >= -1 && lj < TW + 1);
+    return (li + 2) * WSY + (lj + 1);
+  }
+  static HF_HD int ifx(int li, int fj) {
+    HF_BOUNDS_CHECK(li >= -1 && li < TH + 1 && fj >= -1 && fj < TW + 2);
+    return (li + 1) * WFX + (fj + 1);
+  }
+  static HF_HD int ify(int fi, int lj) {
+    HF_BOUNDS_CHECK(fi >= -1 && fi < TH + 2 && lj >= -1 && lj < TW + 1);
+    return (fi + 1) * WFY + (lj + 1);
+  }
+  static HF_HD int ia(int li, int lj) {
+    HF_BOUNDS_CHECK(li >= -1 && li < TH + 1 && lj >= -1 && lj < TW + 1);
+    return (li + 1) * WA + (lj + 1);
+  }
+};
+
+// ------------------------------------------------------------ the stage
+template <class T, int TW, int TH> struct Stage {
+  using S = TileSmem<T, TW, TH>;
+
+  // Face bed of x-face fj in local row li (update.rs:676-700).
+  static HF_HD T zface_x(const S& s, const StageArgs<T>& a, int64_t gi0, int64_t gj0, int li, int fj) {
+    const int64_t gjf = gj0 + fj;
+    if (gjf == 0) {
+      T zc = s.z[S::i3(li, fj)];
+      T zn = (a.nc >= 2) ? s.z[S::i3(li, fj + 1)] : zc;
+      return (ghost_bed(a.bc_kind[SIDE_W], zc, zn) + zc) * 0.5;
+    }
+    if (gjf == a.nc) {
+      T zc = s.z[S::i3(li, fj - 1)];
+      T zn = (a.nc >= 2) ? s.z[S::i3(li, fj - 2)] : zc;
+      return (zc + ghost_bed(a.bc_kind[SIDE_E], zc, zn)) * 0.5;
+    }
+    return interior_z_face(s.dry[S::i4(li, fj - 1)] != 0, s.z[S::i3(li, fj - 1)],
+                           s.dry[S::i4(li, fj)] != 0, s.z[S::i3(li, fj)]);
+  }
+  // Face bed of y-face fi in local column lj (update.rs:704-728).
+  static HF_HD T zface_y(const S& s, const StageArgs<T>& a, int64_t gi0, int64_t gj0, int fi, int lj) {
+    const int64_t gif = gi0 + fi;
+    if (gif == 0) {
+      T zc = s.z[S::i3(fi, lj)];
+      T zn = (a.nr >= 2) ? s.z[S::i3(fi + 1, lj)] : zc;
+      return (ghost_bed(a.bc_kind[SIDE_N], zc, zn) + zc) * 0.5;
+    }
+    if (gif == a.nr) {
+      T zc = s.z[S::i3(fi - 1, lj)];
+      T zn = (a.nr >= 2) ? s.z[S::i3(fi - 2, lj)] : zc;
+      return (zc + ghost_bed(a.bc_kind[SIDE_S], zc, zn)) * 0.5;
+    }
+    return interior_z_face(s.dry[S::i4(fi - 1, lj)] != 0, s.z[S::i3(fi - 1, lj)],
+                           s.dry[S::i4(fi, lj)] != 0, s.z[S::i3(fi, lj)]);
+  }
+
+  // Phase 1: dry flags on the 4-halo; h, z and primitives u, v on the
+  // 3-halo (update.rs:405-426, 853-855). Cells outside the domain are
+  // "not dry", which clips the +-2 dry windows to the domain as the
+  // reference does.
+  static HF_HD void phase_load(S& s, const StageArgs<T>& a, int64_t gi0, int64_t gj0, int tid, int nt) {
+    const T Z = cst<T>(0.0);
+    for (int k = tid; k < S::N4; k += nt) {
+      const int li = k / S::W4 - 4, lj = k % S::W4 - 4;
+      const int64_t gi = gi0 + li, gj = gj0 + lj;
+      const bool inside = gi >= 0 && gi < a.nr && gj >= 0 && gj < a.nc;
+      const bool in3 = li >= -3 && li < TH + 3 && lj >= -3 && lj < TW + 3;
+      if (inside) {
+        const int64_t idx = gi * a.nc + gj;
+        T hh = a.ih.ld(idx);
+        s.dry[k] = val(hh) <= H_DRY;
+        if (in3) {
+          const int k3 = S::i3(li, lj);
+          s.h[k3] = hh;
+          s.z[k3] = a.z.ld(idx);
+          if (val(hh) > H_VEL) {
+            s.u[k3] = a.ihu.ld(idx) / hh;
+            s.v[k3] = a.ihv.ld(idx) / hh;
+          } else {
+            s.u[k3] = Z;
+            s.v[k3] = Z;
+          }
+        }
+      } else {
+        s.dry[k] = 0;
+        if (in3) {
+          const int k3 = S::i3(li, lj);
+          s.h[k3] = Z;
+          s.z[k3] = Z;
+          s.u[k3] = Z;
+          s.v[k3] = Z;
+        }
+      }
+    }
+  }
+
+  // Phase 2: minmod slopes of (eta, u, v) in x and y (update.rs:459-540).
+  static HF_HD void phase_slopes(S& s, const StageArgs<T>& a, int64_t gi0, int64_t gj0, int tid, int nt) {
+    const T Z = cst<T>(0.0);
+    const double dx = a.dx, dy = a.dy;
+    for (int k = tid; k < S::NSX; k += nt) {
+      const int li = k / S::WSX - 1, lj = k % S::WSX - 2;
+      const int64_t gi = gi0 + li, gj = gj0 + lj;
+      T se = Z, su = Z, sv = Z;
+      if (gi >= 0 && gi < a.nr && gj >= 0 && gj < a.nc && a.nc >= 2) {
+        bool near_dry = false;
+        for (int d = -2; d <= 2; ++d) near_dry = near_dry || (s.dry[S::i4(li, lj + d)] != 0);
+        if (!near_dry) {
+          const int c = S::i3(li, lj);
+          if (gj == 0) {
+            const int r = S::i3(li, lj + 1);
+            se = (s.h[r] + s.z[r] - (s.h[c] + s.z[c])) / dx;
+            su = (s.u[r] - s.u[c]) / dx;
+            sv = (s.v[r] - s.v[c]) / dx;
+          } else if (gj + 1 == a.nc) {
+            const int l = S::i3(li, lj - 1);
+            se = (s.h[c] + s.z[c] - (s.h[l] + s.z[l])) / dx;
+            su = (s.u[c] - s.u[l]) / dx;
+            sv = (s.v[c] - s.v[l]) / dx;
+          } else {
+            const int l = S::i3(li, lj - 1), r = S::i3(li, lj + 1);
+            T el = s.h[l] + s.z[l], ec = s.h[c] + s.z[c], er = s.h[r] + s.z[r];
+            se = minmod((ec - el) / dx, (er - ec) / dx);
+            su = minmod((s.u[c] - s.u[l]) / dx, (s.u[r] - s.u[c]) / dx);
+            sv = minmod((s.v[c] - s.v[l]) / dx, (s.v[r] - s.v[c]) / dx);
+          }
+        }
+      }
+      s.sxe[k] = se;
+      s.sxu[k] = su;
+      s.sxv[k] = sv;
+    }
+    for (int k = tid; k < S::NSY; k += nt) {
+      const int li = k / S::WSY - 2, lj = k % S::WSY - 1;
+      const int64_t gi = gi0 + li, gj = gj0 + lj;
+      T se = Z, su = Z, sv = Z;
+      if (gi >= 0 && gi < a.nr && gj >= 0 && gj < a.nc && a.nr >= 2) {
+        bool near_dry = false;
+        for (int d = -2; d <= 2; ++d) near_dry = near_dry || (s.dry[S::i4(li + d, lj)] != 0);
+        if (!near_dry) {
+          const int c = S::i3(li, lj);
+          if (gi == 0) {
+            const int b = S::i3(li + 1, lj);
+            se = (s.h[b] + s.z[b] - (s.h[c] + s.z[c])) / dy;
+            su = (s.u[b] - s.u[c]) / dy;
+            sv = (s.v[b] - s.v[c]) / dy;
+          } else if (gi + 1 == a.nr) {
+            const int t = S::i3(li - 1, lj);
+            se = (s.h[c] + s.z[c] - (s.h[t] + s.z[t])) / dy;
+            su = (s.u[c] - s.u[t]) / dy;
+            sv = (s.v[c] - s.v[t]) / dy;
+          } else {
+            const int t = S::i3(li - 1, lj), b = S::i3(li + 1, lj);
+            T et = s.h[t] + s.z[t], ec = s.h[c] + s.z[c], eb = s.h[b] + s.z[b];
+            se = minmod((ec - et) / dy, (eb - ec) / dy);
+            su = minmod((s.u[c] - s.u[t]) / dy, (s.u[b] - s.u[c]) / dy);
+            sv = minmod((s.v[c] - s.v[t]) / dy, (s.v[b] - s.v[c]) / dy);
+          }
+        }
+      }
+      s.sye[k] = se;
+      s.syu[k] = su;
+      s.syv[k] = sv;
+    }
+  }
+
+  // Raw state of a domain cell (boundary faces use it unreconstructed,
+  // update.rs:892-897).
+  static HF_HD void raw(const S& s, const StageArgs<T>& a, int li, int lj, int64_t gi, int64_t gj, T* h,
+                        T* hu, T* hv) {
+    const int64_t idx = gi * a.nc + gj;
+    *h = s.h[S::i3(li, lj)];
+    *hu = a.ihu.ld(idx);
+    *hv = a.ihv.ld(idx);
+  }
+
+  // Phase 3: every x- and y-face of the tile and its 1-cell ring
+  // (update.rs:560-623, 890-922).
+  static HF_HD void phase_faces(S& s, const StageArgs<T>& a, int64_t gi0, int64_t gj0, int tid, int nt) {
+    const T Z = cst<T>(0.0);
+    const double half_dx = 0.5 * a.dx, half_dy = 0.5 * a.dy;
+    for (int k = tid; k < S::NFX; k += nt) {
+      const int li = k / S::WFX - 1, fj = k % S::WFX - 1;
+      const int64_t gi = gi0 + li, gjf = gj0 + fj;
+      T m = Z, na = Z, nb = Z, tg = Z;
+      if (gi >= 0 && gi < a.nr && gjf >= 0 && gjf <= a.nc) {
+        T zf = zface_x(s, a, gi0, gj0, li, fj);
+        if (gjf == 0 || gjf == a.nc) {
+          const bool west = (gjf == 0);
+          const int lc = west ? fj : fj - 1;
+          const int ln = west ? fj + 1 : fj - 2;
+          const int64_t gc = west ? 0 : a.nc - 1;
+          T h, hu, hv, gh, ghu, ghv;
+          raw(s, a, li, lc, gi, gc, &h, &hu, &hv);
+          T zc = s.z[S::i3(li, lc)];
+          T zn = (a.nc >= 2) ? s.z[S::i3(li, ln)] : zc;
+          const int side = west ? SIDE_W : SIDE_E;
+          ghost_state(a.bc_kind[side], a.bc_param[side], side, h, hu, hv, zc, zn, a.nc >= 2, a.dx,
+                      a.man.ld(gi * a.nc + gc), &gh, &ghu, &ghv);
+          if (west)
+            well_balanced_face(gh, ghu, ghv, h, hu, hv, zf, &m, &na, &nb, &tg);
+          else
+            well_balanced_face(h, hu, hv, gh, ghu, ghv, zf, &m, &na, &nb, &tg);
+        } else if (!(s.dry[S::i4(li, fj - 1)] && s.dry[S::i4(li, fj)])) {
+          const int l = S::i3(li, fj - 1), r = S::i3(li, fj);
+          const int sl = S::isx(li, fj - 1), sr = S::isx(li, fj);
+          T eta_m = s.h[l] + s.z[l] + s.sxe[sl] * half_dx;
+          T u_m = s.u[l] + s.sxu[sl] * half_dx;
+          T v_m = s.v[l] + s.sxv[sl] * half_dx;
+          T eta_p = s.h[r] + s.z[r] - s.sxe[sr] * half_dx;
+          T u_p = s.u[r] - s.sxu[sr] * half_dx;
+          T v_p = s.v[r] - s.sxv[sr] * half_dx;
+          T h_m = rmax(eta_m - zf, Z);
+          T h_p = rmax(eta_p - zf, Z);
+          well_balanced_face(h_m, h_m * u_m, h_m * v_m, h_p, h_p * u_p, h_p * v_p, zf, &m, &na, &nb, &tg);
+        }
+      }
+      s.fxm[k] = m;
+      s.fxa[k] = na;
+      s.fxb[k] = nb;
+      s.fxt[k] = tg;
+    }
+    for (int k = tid; k < S::NFY; k += nt) {
+      const int fi = k / S::WFY - 1, lj = k % S::WFY - 1;
+      const int64_t gif = gi0 + fi, gj = gj0 + lj;
+      T m = Z, na = Z, nb = Z, tg = Z;
+      if (gj >= 0 && gj < a.nc && gif >= 0 && gif <= a.nr) {
+        T zf = zface_y(s, a, gi0, gj0, fi, lj);
+        if (gif == 0 || gif == a.nr) {
+          const bool north = (gif == 0);
+          const int lc = north ? fi : fi - 1;
+          const int ln = north ? fi + 1 : fi - 2;
+          const int64_t gc = north ? 0 : a.nr - 1;
+          T h, hu, hv, gh, ghu, ghv;
+          raw(s, a, lc, lj, gc, gj, &h, &hu, &hv);
+          T zc = s.z[S::i3(lc, lj)];
+          T zn = (a.nr >= 2) ? s.z[S::i3(ln, lj)] : zc;
+          const int side = north ? SIDE_N : SIDE_S;
+          ghost_state(a.bc_kind[side], a.bc_param[side], side, h, hu, hv, zc, zn, a.nr >= 2, a.dy,
+                      a.man.ld(gc * a.nc + gj), &gh, &ghu, &ghv);
+          // y-face: normal momentum is hv, tangential hu (riemann.rs:97-107)
+          if (north)
+            well_balanced_face(gh, ghv, ghu, h, hv, hu, zf, &m, &na, &nb, &tg);
+          else
+            well_balanced_face(h, hv, hu, gh, ghv, ghu, zf, &m, &na, &nb, &tg);
+        } else if (!(s.dry[S::i4(fi - 1, lj)] && s.dry[S::i4(fi, lj)])) {
+          const int t = S::i3(fi - 1, lj), b = S::i3(fi, lj);
+          const int st = S::isy(fi - 1, lj), sb = S::isy(fi, lj);
+          T eta_m = s.h[t] + s.z[t] + s.sye[st] * half_dy;
+          T u_m = s.u[t] + s.syu[st] * half_dy;
+          T v_m = s.v[t] + s.syv[st] * half_dy;
+          T eta_p = s.h[b] + s.z[b] - s.sye[sb] * half_dy;
+          T u_p = s.u[b] - s.syu[sb] * half_dy;
+          T v_p = s.v[b] - s.syv[sb] * half_dy;
+          T h_m = rmax(eta_m - zf, Z);
+          T h_p = rmax(eta_p - zf, Z);
+          well_balanced_face(h_m, h_m * v_m, h_m * u_m, h_p, h_p * v_p, h_p * u_p, zf, &m, &na, &nb, &tg);
+        }
+      }
+      s.fym[k] = m;
+      s.fya[k] = na;
+      s.fyb[k] = nb;
+      s.fyt[k] = tg;
+    }
+  }
+
+  // Phase 4: Liang-Marche outflow limiter alpha on the tile and its ring
+  // (update.rs:953-982).
+  static HF_HD void phase_alpha(S& s, const StageArgs<T>& a, int64_t gi0, int64_t gj0, int tid, int nt,
+                                double dt) {
+    const double dt_dx = dt / a.dx, dt_dy = dt / a.dy;
+    const T Z = cst<T>(0.0);
+    for (int k = tid; k < S::NA; k += nt) {
+      const int li = k / S::WA - 1, lj = k % S::WA - 1;
+      const int64_t gi = gi0 + li, gj = gj0 + lj;
+      T al = cst<T>(1.0);
+      if (gi >= 0 && gi < a.nr && gj >= 0 && gj < a.nc) {
+        T fx_right = s.fxm[S::ifx(li, lj + 1)];
+        T fx_left = s.fxm[S::ifx(li, lj)];
+        T fy_bottom = s.fym[S::ify(li + 1, lj)];
+        T fy_top = s.fym[S::ify(li, lj)];
+        T out = Z;
+        if (val(fx_right) > 0.0) out = out + fx_right * dt_dx;
+        if (val(fx_left) < 0.0) out = out + (-fx_left) * dt_dx;
+        if (val(fy_bottom) > 0.0) out = out + fy_bottom * dt_dy;
+        if (val(fy_top) < 0.0) out = out + (-fy_top) * dt_dy;
+        T avail = rmax(s.h[S::i3(li, lj)] - H_DRY, Z);
+        if (val(out) > val(avail) && val(out) > 0.0) al = avail / out;
+      }
+      s.alpha[k] = al;
+    }
+  }
+
+  // Upstream-alpha scaling of one x-face as seen from local row li
+  // (update.rs:992-1025). Dry-dry interior faces are left as computed.
+  static HF_HD void scaled_fx(const S& s, const StageArgs<T>& a, int64_t gj0, int li, int fj, T* m, T* na,
+                              T* nb, T* tg) {
+    const int k = S::ifx(li, fj);
+    *m = s.fxm[k];
+    *na = s.fxa[k];
+    *nb = s.fxb[k];
+    *tg = s.fxt[k];
+    const int64_t gjf = gj0 + fj;
+    if (gjf > 0 && gjf < a.nc && s.dry[S::i4(li, fj - 1)] && s.dry[S::i4(li, fj)]) return;
+    T al;
+    if (gjf == 0)
+      al = val(*m) < 0.0 ? s.alpha[S::ia(li, fj)] : cst<T>(1.0);
+    else if (gjf == a.nc)
+      al = val(*m) > 0.0 ? s.alpha[S::ia(li, fj - 1)] : cst<T>(1.0);
+    else if (val(*m) > 0.0)
+      al = s.alpha[S::ia(li, fj - 1)];
+    else
+      al = s.alpha[S::ia(li, fj)];
+    *m = *m * al;
+    *na = *na * al;
+    *nb = *nb * al;
+    *tg = *tg * al;
+  }
+  // update.rs:1026-1053
+  static HF_HD void scaled_fy(const S& s, const StageArgs<T>& a, int64_t gi0, int fi, int lj, T* m, T* na,
+                              T* nb, T* tg) {
+    const int k = S::ify(fi, lj);
+    *m = s.fym[k];
+    *na = s.fya[k];
+    *nb = s.fyb[k];
+    *tg = s.fyt[k];
+    const int64_t gif = gi0 + fi;
+    if (gif > 0 && gif < a.nr && s.dry[S::i4(fi - 1, lj)] && s.dry[S::i4(fi, lj)]) return;
+    T al;
+    if (gif == 0)
+      al = val(*m) < 0.0 ? s.alpha[S::ia(fi, lj)] : cst<T>(1.0);
+    else if (gif == a.nr)
+      al = val(*m) > 0.0 ? s.alpha[S::ia(fi - 1, lj)] : cst<T>(1.0);
+    else if (val(*m) > 0.0)
+      al = s.alpha[S::ia(fi - 1, lj)];
+    else
+      al = s.alpha[S::ia(fi, lj)];
+    *m = *m * al;
+    *na = *na * al;
+    *nb = *nb * al;
+    *tg = *tg * al;
+  }
+
+  // Wave speeds of one cell for the CFL bound (update.rs:113-136) and, on
+  // domain edges, of its ghosts (update.rs:179-201, no H_VEL cut there).
+  static HF_HD void cell_wave_speed(const StageArgs<T>& a, int64_t gi, int64_t gj, T h, T hu, T hv, T zc,
+                                    const T* zn4, const T* man, double* sx, double* sy) {
+    const double hv_ = val(h);
+    if (hv_ > H_DRY) {
+      double c = __builtin_sqrt(GRAVITY * hv_);
+      double u = 0.0, v = 0.0;
+      if (hv_ > H_VEL) {
+        u = val(hu) / hv_;
+        v = val(hv) / hv_;
+      }
+      *sx = rmax(*sx, __builtin_fabs(u) + c);
+      *sy = rmax(*sy, __builtin_fabs(v) + c);
+    }
+    if (!a.cfl_with_bcs) return;
+    const bool edge[4] = {gi == 0, gi == a.nr - 1, gj == 0, gj == a.nc - 1};
+    for (int side = 0; side < 4; ++side) {
+      if (!edge[side]) continue;
+      const bool xf = (side == SIDE_W || side == SIDE_E);
+      const bool have_next = xf ? (a.nc >= 2) : (a.nr >= 2);
+      T gh, ghu, ghv;
+      ghost_state(a.bc_kind[side], a.bc_param[side], side, h, hu, hv, zc, zn4[side], have_next,
+                  xf ? a.dx : a.dy, *man, &gh, &ghu, &ghv);
+      const double g = val(gh);
+      if (g <= H_DRY) continue;
+      double c = __builtin_sqrt(GRAVITY * g);
+      double u = val(ghu) / g;
+      double v = val(ghv) / g;
+      *sx = rmax(*sx, __builtin_fabs(u) + c);
+      *sy = rmax(*sy, __builtin_fabs(v) + c);
+    }
+  }
+
+  // Cell-local operator-split tail in the reference callers' order
+  // (sim.rs:224; uk_ea_test8a_official.rs:206-217): Manning friction
+  // (source.rs:130-165), rain (source.rs:88-101), point sources
+  // (source.rs:52-69).
+  static HF_HD void forcing(const StageArgs<T>& a, int64_t gi, int64_t gj, double dt, T* h, T* hu, T* hv) {
+    const T Z = cst<T>(0.0);
+    if (a.friction && val(*h) > a.friction_dry_tol) {
+      T n = a.man.ld(gi * a.nc + gj);
+      if (val(n) != 0.0) {
+        const double dt_g = dt * GRAVITY;
+        T speed = rsqrt_(*hu * *hu + *hv * *hv) / *h;
+        T factor = n * n * speed / pow43(*h) * dt_g + 1.0;
+        *hu = *hu / factor;
+        *hv = *hv / factor;
+      }
+    }
+    if (a.rain_rate != 0.0) {
+      const double dh = a.rain_rate * dt;
+      T nh = *h + dh;
+      if (val(nh) <= 0.0) {
+        *h = Z;
+        *hu = Z;
+        *hv = Z;
+      } else {
+        *h = nh;
+      }
+    }
+    if (a.n_src > 0) {
+      const double cell_area = a.dx * a.dy;
+      for (int k = 0; k < a.n_src; ++k) {
+        if (a.srcs[k].row != gi || a.srcs[k].col != gj) continue;
+        const double dh = a.srcs[k].q * dt / cell_area;
+        *h = rmax(*h + dh, Z);
+        if (val(*h) == 0.0) {
+          *hu = Z;
+          *hv = Z;
+        }
+      }
+    }
+  }
+
+  // Phase 5: FV update with the explicit bed-slope source
+  // (update.rs:1071-1210), then the stage epilogue: write U^(1), or the
+  // SSP-RK2 combine (update.rs:1272-1286) + forcing + CFL wave speeds.
+  template <int MODE>
+  static HF_HD void phase_update(S& s, const StageArgs<T>& a, int64_t gi0, int64_t gj0, int tid, int nt,
+                                 double dt, double* wsx, double* wsy) {
+    const T Z = cst<T>(0.0);
+    const double dt_dx = dt / a.dx, dt_dy = dt / a.dy;
+    for (int k = tid; k < TW * TH; k += nt) {
+      const int li = k / TW, lj = k % TW;
+      const int64_t gi = gi0 + li, gj = gj0 + lj;
+      if (gi >= a.nr || gj >= a.nc) continue;
+      const int64_t idx = gi * a.nc + gj;
+      const int c3 = S::i3(li, lj);
+      const T h_old = s.h[c3];
+      T nh, nhu, nhv;
+      if (s.dry[S::i4(li, lj)] && gi > 0 && gi + 1 < a.nr && gj > 0 && gj + 1 < a.nc &&
+          s.dry[S::i4(li - 1, lj)] && s.dry[S::i4(li + 1, lj)] && s.dry[S::i4(li, lj - 1)] &&
+          s.dry[S::i4(li, lj + 1)]) {
+        nh = h_old;
+        nhu = Z;
+        nhv = Z;
+      } else {
+        T rm, ra, rb, rt, lm, la, lb, lt, bm, ba, bb, bt, tm, ta, tb, tt;
+        scaled_fx(s, a, gj0, li, lj + 1, &rm, &ra, &rb, &rt);
+        scaled_fx(s, a, gj0, li, lj, &lm, &la, &lb, &lt);
+        scaled_fy(s, a, gi0, li + 1, lj, &bm, &ba, &bb, &bt);
+        scaled_fy(s, a, gi0, li, lj, &tm, &ta, &tb, &tt);
+        // right face .minus, left face .plus, bottom face .minus, top face .plus
+        T dh = (rm - lm) * dt_dx + (bm - tm) * dt_dy;
+        T new_h = h_old - dh;
+        T dhu = (ra - lb) * dt_dx + (bt - tt) * dt_dy;
+        T dhv = (rt - lt) * dt_dx + (ba - tb) * dt_dy;
+
+        T eta_cell = h_old + s.z[c3];
+        T sex = s.sxe[S::isx(li, lj)];
+        T eta_left = (gj > 0) ? eta_cell - sex * (0.5 * a.dx) : eta_cell;
+        T eta_right = (gj + 1 < a.nc) ? eta_cell + sex * (0.5 * a.dx) : eta_cell;
+        T hfl = rmax(eta_left - zface_x(s, a, gi0, gj0, li, lj), Z);
+        T hfr = rmax(eta_right - zface_x(s, a, gi0, gj0, li, lj + 1), Z);
+        T zel = eta_left - hfl;
+        T zer = eta_right - hfr;
+        T s_hu = (hfl + hfr) * (zel - zer) * (0.5 * GRAVITY) / a.dx;
+
+        T sey = s.sye[S::isy(li, lj)];
+        T eta_top = (gi > 0) ? eta_cell - sey * (0.5 * a.dy) : eta_cell;
+        T eta_bot = (gi + 1 < a.nr) ? eta_cell + sey * (0.5 * a.dy) : eta_cell;
+        T hft = rmax(eta_top - zface_y(s, a, gi0, gj0, li, lj), Z);
+        T hfb = rmax(eta_bot - zface_y(s, a, gi0, gj0, li + 1, lj), Z);
+        T zet = eta_top - hft;
+        T zeb = eta_bot - hfb;
+        T s_hv = (hft + hfb) * (zet - zeb) * (0.5 * GRAVITY) / a.dy;
+
+        if (val(new_h) <= H_VEL) {
+          nh = rmax(new_h, Z);
+          nhu = Z;
+          nhv = Z;
+        } else {
+          T hu_old = Z, hv_old = Z;
+          if (val(h_old) > H_VEL) {
+            hu_old = a.ihu.ld(idx);
+            hv_old = a.ihv.ld(idx);
+          }
+          nh = new_h;
+          nhu = hu_old - dhu + s_hu * dt;
+          nhv = hv_old - dhv + s_hv * dt;
+        }
+      }
+      if (MODE == MODE_FINAL_RK2) {
+        T ph = a.nh.ld(idx), phu = a.nhu.ld(idx), phv = a.nhv.ld(idx);
+        nh = (ph + nh) * 0.5;
+        nhu = (phu + nhu) * 0.5;
+        nhv = (phv + nhv) * 0.5;
+        if (val(nh) <= H_VEL) {
+          nh = rmax(nh, Z);
+          nhu = Z;
+          nhv = Z;
+        }
+        forcing(a, gi, gj, dt, &nh, &nhu, &nhv);
+        if (a.wave_speed) {
+          T zn4[4];
+          zn4[SIDE_N] = (a.nr >= 2 && gi == 0) ? s.z[S::i3(li + 1, lj)] : s.z[c3];
+          zn4[SIDE_S] = (a.nr >= 2 && gi == a.nr - 1) ? s.z[S::i3(li - 1, lj)] : s.z[c3];
+          zn4[SIDE_W] = (a.nc >= 2 && gj == 0) ? s.z[S::i3(li, lj + 1)] : s.z[c3];
+          zn4[SIDE_E] = (a.nc >= 2 && gj == a.nc - 1) ? s.z[S::i3(li, lj - 1)] : s.z[c3];
+          T man = a.cfl_with_bcs ? a.man.ld(idx) : Z;
+          cell_wave_speed(a, gi, gj, nh, nhu, nhv, s.z[c3], zn4, &man, wsx, wsy);
+        }
+      }
+      a.oh.st(idx, nh);
+      a.ohu.st(idx, nhu);
+      a.ohv.st(idx, nhv);
+    }
+  }
+};
+
+// dt of the step about to run, from the device clock: the CFL bound of
+// the current state (update.rs:170-208), capped by max_dt and dt_cap
+// (sim.rs:206-208).
+HF_HD double dt_from_ctrl(const Ctrl* c, double cfl, double dx, double dy, double max_dt) {
+  const double sx = hfm::as_f64(c->ws[c->parity][0]);
+  const double sy = hfm::as_f64(c->ws[c->parity][1]);
+  const double denom = sx / dx + sy / dy;
+  double dt = denom > 0.0 ? cfl / denom : __builtin_inf();
+  const double cap = c->use_target ? (c->t_target - c->t) : c->dt_cap;
+  dt = rmin(rmin(dt, max_dt), cap);
+  return dt;
+}
+
+// Common prologue of every stage launch. Returns false when the launch is
+// a no-op (driver finished or failed); block 0 records why.
+template <class T>
+HF_HD bool stage_prologue(const StageArgs<T>& a, bool lead, double* dt_out) {
+  if (a.dt_mode == 0) {
+    *dt_out = a.dt_host;
+    return true;
+  }
+  Ctrl* c = a.ctrl;
+  if (c->status != 0 || c->done != 0) return false;
+  if (!a.first_stage) {
+    *dt_out = c->dt;
+    return true;
+  }
+  if (c->use_target && !(c->t < c->t_target)) {  // sim.rs:233
+    if (lead) c->done = 1;
+    return false;
+  }
+  if (c->budget == 0) {  // sim.rs:234-240
+    if (lead) c->status = 4;
+    return false;
+  }
+  double dt = dt_from_ctrl(c, a.cfl, a.dx, a.dy, a.max_dt);
+  if (!(__builtin_isfinite(dt) && dt > 0.0)) {  // sim.rs:209-215
+    if (lead) {
+      c->dt = dt;
+      c->status = 3;
+    }
+    return false;
+  }
+  if (lead) {
+    c->dt = dt;
+    c->ws[c->parity ^ 1][0] = 0ULL;
+    c->ws[c->parity ^ 1][1] = 0ULL;
+  }
+  *dt_out = dt;
+  return true;
+}
+
+}  // namespace hf

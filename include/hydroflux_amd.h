@@ -1,0 +1,170 @@
+/*
+ * hydroflux_amd: C ABI of the MI355X (gfx950) solver-2d second-order step.
+ *
+ * Drop-in boundary for hydroflux's solver-2d time step. Each entry point
+ * replaces one item of the reference's public Rust API (solver-2d/src/lib.rs:
+ * 74-91); the comment on each names the reference item (file:line) whose
+ * behaviour it reproduces. Grids stay resident in device memory between
+ * calls; host arrays cross the boundary only at upload/download.
+ *
+ * Layout at the boundary: row-major (n_rows, n_cols), row i along y, column
+ * j along x (solver-2d/src/lib.rs:11-25). States are AoS float64 triples
+ * (h, hu, hv) per cell, the logical order of the reference's
+ * Array2<Conserved2D>; the Dual kind takes a second array of the same shape
+ * holding the tangents (dval) of (h, hu, hv). Plain pointers and sizes only.
+ *
+ * Errors are returned as int status codes instead of the reference's panics
+ * and SimError values (sim.rs:81-115); hf2d_last_error() gives the message.
+ * One handle per host thread; calls on one handle are serialised by the
+ * caller.
+ */
+#ifndef HYDROFLUX_AMD_H
+#define HYDROFLUX_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes. 1-4 map 1:1 onto SimError (sim.rs:81-115). */
+enum hf2d_status {
+  HF2D_OK = 0,
+  HF2D_SHAPE_MISMATCH = 1,         /* SimError::ShapeMismatch / shape panics */
+  HF2D_INVALID_CONFIG = 2,         /* SimError::InvalidConfig / Mesh2DG::new panics */
+  HF2D_DEGENERATE_DT = 3,          /* SimError::DegenerateDt */
+  HF2D_STEP_BUDGET_EXHAUSTED = 4,  /* SimError::StepBudgetExhausted */
+  HF2D_HIP_ERROR = 5,              /* device runtime failure */
+  HF2D_INVALID_ARGUMENT = 6        /* null handle / pointer, unsupported kind */
+};
+
+/* Scalar of the grid: f64, or forward-mode Dual (autograd/src/dual.rs:12). */
+enum hf2d_scalar_kind { HF2D_F64 = 0, HF2D_DUAL = 1 };
+
+/* Boundary kinds (boundary.rs:80-106). Sides are ordered N, S, W, E
+ * (Boundaries2D, boundary.rs:131-140). param is q for Discharge, h for
+ * Depth, ignored otherwise. */
+enum hf2d_bc_kind {
+  HF2D_BC_TRANSMISSIVE = 0,
+  HF2D_BC_WALL = 1,
+  HF2D_BC_DISCHARGE = 2,
+  HF2D_BC_DEPTH = 3
+};
+
+/* Integrator (sim.rs:37-46). */
+enum hf2d_integrator { HF2D_FORWARD_EULER = 0, HF2D_SSPRK2 = 1 };
+
+/* PointSource (source.rs:36-44). */
+typedef struct hf2d_point_source {
+  int64_t row;
+  int64_t col;
+  double q_mass; /* m^3/s; negative is a sink */
+} hf2d_point_source;
+
+/* SimulationConfig (sim.rs:49-77) plus the forcing the reference's
+ * operational callers apply after each step (uk_ea_test8a_official.rs:
+ * 206-217): friction dry tolerance (Simulation uses H_DRY, sim.rs:224),
+ * uniform rain rate [m/s], and the point sources passed to hf2d_configure. */
+typedef struct hf2d_config {
+  double cfl;              /* (0, 1]; default 0.4 */
+  int32_t integrator;      /* hf2d_integrator; default SSP-RK2 */
+  int32_t cfl_with_bcs;    /* 1: cfl_time_step_with_bcs (Simulation); 0: cfl_time_step */
+  double max_dt;           /* > 0; default 60 s */
+  int32_t friction;        /* apply manning_friction_step after the integrator */
+  int32_t reserved;
+  double friction_dry_tol; /* default H_DRY = 1e-6 */
+  int64_t max_steps;       /* run_until budget; default 1e7 */
+  double rain_rate;        /* apply_rain rate [m/s] after friction; 0 disables */
+} hf2d_config;
+
+typedef struct hf2d_sim hf2d_sim;
+
+/* Mesh2DG::new / StepWorkspace2D::new (geometry.rs:58-76, update.rs:756-771).
+ * Fails with HF2D_INVALID_CONFIG on dx <= 0, dy <= 0 or an empty grid. */
+int hf2d_create(int64_t n_rows, int64_t n_cols, double dx, double dy, int32_t scalar_kind,
+                hf2d_sim** out);
+int hf2d_destroy(hf2d_sim* sim);
+const char* hf2d_last_error(const hf2d_sim* sim);
+
+/* Mesh2DG::with_manning_field (geometry.rs:90-113): bed z and Manning n per
+ * cell. Tangent arrays (Dual kind only) may be null (tangent 0). Negative n
+ * fails with HF2D_INVALID_CONFIG. */
+int hf2d_upload_mesh(hf2d_sim* sim, const double* bed, const double* bed_tangent,
+                     const double* manning, const double* manning_tangent);
+
+/* The caller-owned Array2<Conserved2DG<T>> of the reference: copy in / out. */
+int hf2d_upload_state(hf2d_sim* sim, const double* state_aos, const double* tangent_aos);
+int hf2d_download_state(hf2d_sim* sim, double* state_aos, double* tangent_aos);
+
+/* Boundaries2D by value (boundary.rs:131-140); order N, S, W, E. */
+int hf2d_set_boundaries(hf2d_sim* sim, const int32_t kinds[4], const double params[4]);
+
+/* max_wave_speeds (update.rs:113-136). */
+int hf2d_max_wave_speeds(hf2d_sim* sim, double* s_x, double* s_y);
+
+/* cfl_time_step (update.rs:148-156) or, with with_bcs != 0,
+ * cfl_time_step_with_bcs (update.rs:170-208). +inf for a dry domain. */
+int hf2d_cfl_time_step(hf2d_sim* sim, double cfl, int32_t with_bcs, double* dt);
+
+/* forward_euler_step_with (update.rs:805-1211) with caller dt. */
+int hf2d_forward_euler_step(hf2d_sim* sim, double dt);
+
+/* ssprk2_step_with (update.rs:1255-1287) with caller dt. */
+int hf2d_ssprk2_step(hf2d_sim* sim, double dt);
+
+/* manning_friction_step (source.rs:130-165). */
+int hf2d_manning_friction_step(hf2d_sim* sim, double dt, double dry_tol);
+
+/* apply_rain (source.rs:88-101). */
+int hf2d_apply_rain(hf2d_sim* sim, double rate, double dt);
+
+/* apply_point_sources (source.rs:52-69), applied in list order. */
+int hf2d_apply_point_sources(hf2d_sim* sim, const hf2d_point_source* sources, int64_t n_sources,
+                             double dt);
+
+/* Simulation::new's checks and config (sim.rs:128-162): cfl in (0, 1],
+ * max_dt > 0, else HF2D_INVALID_CONFIG. Sources are copied. */
+int hf2d_configure(hf2d_sim* sim, const hf2d_config* config, const hf2d_point_source* sources,
+                   int64_t n_sources);
+
+/* Simulation::step (sim.rs:204-228) followed by the configured rain and
+ * point sources. dt = min(CFL, max_dt, dt_cap) is computed on the device;
+ * the dt taken is returned. */
+int hf2d_step(hf2d_sim* sim, double dt_cap, double* dt_taken);
+
+/* Simulation::run_until (sim.rs:231-246). The clock, dt and the step budget
+ * stay on the device; the host only polls for completion. */
+int hf2d_run_until(hf2d_sim* sim, double t_target);
+
+/* n steps of Simulation::step with no cap (a fixed-count loop). */
+int hf2d_run_steps(hf2d_sim* sim, int64_t n_steps);
+
+/* Simulation::time / steps (sim.rs:164-172), and a setter for resuming a
+ * saved run. */
+int hf2d_get_time(hf2d_sim* sim, double* t, int64_t* steps);
+int hf2d_set_time(hf2d_sim* sim, double t, int64_t steps);
+
+/* Measurement: enqueue n device-clocked steps without waiting (no host
+ * round trip per step), and per-kernel HIP-event timing on the handle's
+ * stream. stage_ms[0] / stage_ms[1] are the summed durations of the first
+ * and the final stage kernels over `count` steps since timing was enabled. */
+int hf2d_enqueue_steps(hf2d_sim* sim, int64_t n_steps);
+int hf2d_synchronize(hf2d_sim* sim);
+int hf2d_set_kernel_timing(hf2d_sim* sim, int32_t enabled);
+int hf2d_kernel_timing(hf2d_sim* sim, double stage_ms[2], int64_t* count);
+
+/* Bandwidth reference: average duration of a plain device copy of n_bytes
+ * (read n_bytes, write n_bytes) over `reps` launches. Also the calibration
+ * kernel for hardware traffic counters. */
+int hf2d_copy_probe(int64_t n_bytes, int32_t reps, double* ms_avg);
+
+/* Arithmetic-contract probe: for each x[i] the device's sqrt(x), x / y[i],
+ * pow(x, 4.0/3.0) and pow(x, 4.0/3.0 - 1.0) as used by the step, written to
+ * out[4*i + 0..3]. Lets a test check them bit-for-bit against the host. */
+int hf2d_math_probe(const double* x, const double* y, int64_t n, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HYDROFLUX_AMD_H */

@@ -1,0 +1,413 @@
+// CPU build of the stage-kernel tile code (hydroflux_amd/csrc/hf2d_tile.h).
+//
+// TEST INFRASTRUCTURE ONLY. Runs the same phase functions the gfx950
+// kernels run, one thread index after another per barrier-separated phase
+// (a legal schedule of the kernel), with every tile-local LDS index
+// bounds-checked. Used to check the kernel logic against the oracle on the
+// CPU and to find indexing faults without a GPU. The device driver's
+// sequencing (wave-speed refresh, first stage, final stage, finalize) is
+// mirrored here on a host copy of the control block.
+#include <cstdio>
+#include <cstdlib>
+#include <memory>
+#include <vector>
+
+#define HF_BOUNDS_CHECK(cond)                                                   \
+  do {                                                                          \
+    if (!(cond)) {                                                              \
+      std::fprintf(stderr, "LDS index out of bounds: %s (%s:%d)\n", #cond, __FILE__, __LINE__); \
+      std::abort();                                                             \
+    }                                                                           \
+  } while (0)
+#include "../../hydroflux_amd/csrc/hf2d_tile.h"
+
+using namespace hf;
+
+namespace {
+
+struct Emu {
+  int64_t nr, nc;
+  double dx, dy;
+  int dual_kind;
+  int tw, th;
+  std::vector<double> st[2][3], tg[2][3];
+  std::vector<double> z, zt, man, mant;
+  bool has_zt = false;
+  int cur = 0;
+  Ctrl ctrl{};
+  int bc_kind[4] = {BC_WALL, BC_WALL, BC_WALL, BC_WALL};
+  double bc_param[4] = {0, 0, 0, 0};
+  double cfl = 0.4, max_dt = 60.0, fdt = H_DRY, rain = 0.0;
+  int integrator = 1, with_bcs = 1, friction = 1;
+  int64_t max_steps = 10000000;
+  std::vector<PointSrc> srcs;
+  double tmp_rain = 0.0, tmp_tol = 0.0;
+};
+
+template <class T> Fld<T> fld(const std::vector<double>& v, const std::vector<double>& d);
+template <> Fld<double> fld<double>(const std::vector<double>& v, const std::vector<double>&) {
+  return Fld<double>{v.data()};
+}
+template <> Fld<dual> fld<dual>(const std::vector<double>& v, const std::vector<double>& d) {
+  return Fld<dual>{v.data(), d.empty() ? nullptr : d.data()};
+}
+template <class T> FldW<T> fldw(std::vector<double>& v, std::vector<double>& d);
+template <> FldW<double> fldw<double>(std::vector<double>& v, std::vector<double>&) { return FldW<double>{v.data()}; }
+template <> FldW<dual> fldw<dual>(std::vector<double>& v, std::vector<double>& d) {
+  return FldW<dual>{v.data(), d.data()};
+}
+
+template <class T> StageArgs<T> args(Emu& e, int in, int out, int un, int tw, int th) {
+  StageArgs<T> a{};
+  a.nr = e.nr;
+  a.nc = e.nc;
+  a.dx = e.dx;
+  a.dy = e.dy;
+  a.ih = fld<T>(e.st[in][0], e.tg[in][0]);
+  a.ihu = fld<T>(e.st[in][1], e.tg[in][1]);
+  a.ihv = fld<T>(e.st[in][2], e.tg[in][2]);
+  a.oh = fldw<T>(e.st[out][0], e.tg[out][0]);
+  a.ohu = fldw<T>(e.st[out][1], e.tg[out][1]);
+  a.ohv = fldw<T>(e.st[out][2], e.tg[out][2]);
+  a.nh = fld<T>(e.st[un][0], e.tg[un][0]);
+  a.nhu = fld<T>(e.st[un][1], e.tg[un][1]);
+  a.nhv = fld<T>(e.st[un][2], e.tg[un][2]);
+  static const std::vector<double> none;
+  a.z = fld<T>(e.z, e.has_zt ? e.zt : none);
+  a.man = fld<T>(e.man, e.mant);
+  for (int k = 0; k < 4; ++k) {
+    a.bc_kind[k] = e.bc_kind[k];
+    a.bc_param[k] = e.bc_param[k];
+  }
+  a.ctrl = &e.ctrl;
+  a.cfl = e.cfl;
+  a.max_dt = e.max_dt;
+  a.cfl_with_bcs = e.with_bcs;
+  a.srcs = e.srcs.data();
+  a.tiles_x = (int)((e.nc + tw - 1) / tw);
+  a.tiles_y = (int)((e.nr + th - 1) / th);
+  return a;
+}
+
+// One launch of the stage kernel: every tile, every phase, every thread.
+template <class T, int TW, int TH>
+void launch(const StageArgs<T>& a, int mode, unsigned long long* dst) {
+  using St = Stage<T, TW, TH>;
+  std::unique_ptr<typename St::S> s(new typename St::S());
+  const int NT = 256;
+  const int64_t nb = (int64_t)a.tiles_x * a.tiles_y;
+  double gx = 0.0, gy = 0.0;
+  for (int64_t b = 0; b < nb; ++b) {
+    double dt;
+    if (!stage_prologue(a, b == 0, &dt)) return;
+    const int64_t gi0 = (b / a.tiles_x) * TH, gj0 = (b % a.tiles_x) * TW;
+    for (int t = 0; t < NT; ++t) St::phase_load(*s, a, gi0, gj0, t, NT);
+    for (int t = 0; t < NT; ++t) St::phase_slopes(*s, a, gi0, gj0, t, NT);
+    for (int t = 0; t < NT; ++t) St::phase_faces(*s, a, gi0, gj0, t, NT);
+    for (int t = 0; t < NT; ++t) St::phase_alpha(*s, a, gi0, gj0, t, NT, dt);
+    for (int t = 0; t < NT; ++t) {
+      double wx = 0.0, wy = 0.0;
+      if (mode == MODE_STAGE)
+        St::template phase_update<MODE_STAGE>(*s, a, gi0, gj0, t, NT, dt, &wx, &wy);
+      else
+        St::template phase_update<MODE_FINAL_RK2>(*s, a, gi0, gj0, t, NT, dt, &wx, &wy);
+      gx = rmax(gx, wx);
+      gy = rmax(gy, wy);
+    }
+  }
+  if (dst && mode == MODE_FINAL_RK2 && a.wave_speed) {
+    if (hfm::as_u64(gx) > dst[0]) dst[0] = hfm::as_u64(gx);
+    if (hfm::as_u64(gy) > dst[1]) dst[1] = hfm::as_u64(gy);
+  }
+}
+
+// Elementwise tail (forcing + CFL maxima), as tail_kernel.
+template <class T> void tail(const StageArgs<T>& a, unsigned long long* dst) {
+  using St = Stage<T, 32, 16>;
+  double dt;
+  if (!stage_prologue(a, true, &dt)) return;
+  double wx = 0.0, wy = 0.0;
+  for (int64_t idx = 0; idx < a.nr * a.nc; ++idx) {
+    const int64_t gi = idx / a.nc, gj = idx % a.nc;
+    T h = a.ih.ld(idx), hu = a.ihu.ld(idx), hv = a.ihv.ld(idx);
+    St::forcing(a, gi, gj, dt, &h, &hu, &hv);
+    a.oh.st(idx, h);
+    a.ohu.st(idx, hu);
+    a.ohv.st(idx, hv);
+    if (a.wave_speed) {
+      T zc = a.z.ld(idx);
+      T zn4[4];
+      zn4[SIDE_N] = (a.nr >= 2 && gi == 0) ? a.z.ld(idx + a.nc) : zc;
+      zn4[SIDE_S] = (a.nr >= 2 && gi == a.nr - 1) ? a.z.ld(idx - a.nc) : zc;
+      zn4[SIDE_W] = (a.nc >= 2 && gj == 0) ? a.z.ld(idx + 1) : zc;
+      zn4[SIDE_E] = (a.nc >= 2 && gj == a.nc - 1) ? a.z.ld(idx - 1) : zc;
+      T man = a.man.ld(idx);
+      St::cell_wave_speed(a, gi, gj, h, hu, hv, zc, zn4, &man, &wx, &wy);
+    }
+  }
+  if (a.wave_speed && dst) {
+    if (hfm::as_u64(wx) > dst[0]) dst[0] = hfm::as_u64(wx);
+    if (hfm::as_u64(wy) > dst[1]) dst[1] = hfm::as_u64(wy);
+  }
+}
+
+template <class T, int TW, int TH> void refresh(Emu& e, bool with_bcs) {
+  e.ctrl.ws[e.ctrl.parity][0] = 0;
+  e.ctrl.ws[e.ctrl.parity][1] = 0;
+  StageArgs<T> a = args<T>(e, e.cur, e.cur, e.cur, TW, TH);
+  a.dt_mode = 0;
+  a.wave_speed = 1;
+  a.cfl_with_bcs = with_bcs;
+  tail<T>(a, e.ctrl.ws[e.ctrl.parity]);
+}
+
+template <class T, int TW, int TH> void host_dt_step(Emu& e, double dt, bool rk2) {
+  const int cur = e.cur, oth = 1 - cur;
+  StageArgs<T> a = args<T>(e, cur, oth, cur, TW, TH);
+  a.dt_mode = 0;
+  a.dt_host = dt;
+  launch<T, TW, TH>(a, MODE_STAGE, nullptr);
+  if (rk2) {
+    StageArgs<T> b = args<T>(e, oth, cur, cur, TW, TH);
+    b.dt_mode = 0;
+    b.dt_host = dt;
+    launch<T, TW, TH>(b, MODE_FINAL_RK2, nullptr);
+  } else {
+    e.cur = oth;
+  }
+}
+
+template <class T, int TW, int TH> void driver_step(Emu& e) {
+  const int cur = e.cur, oth = 1 - cur;
+  StageArgs<T> a = args<T>(e, cur, oth, cur, TW, TH);
+  a.dt_mode = 1;
+  a.first_stage = 1;
+  StageArgs<T> b = args<T>(e, oth, cur, cur, TW, TH);
+  b.dt_mode = 1;
+  b.first_stage = 0;
+  b.friction = e.friction;
+  b.friction_dry_tol = e.fdt;
+  b.rain_rate = e.rain;
+  b.n_src = (int)e.srcs.size();
+  b.wave_speed = 1;
+  launch<T, TW, TH>(a, MODE_STAGE, nullptr);
+  if (e.integrator == 1) {
+    launch<T, TW, TH>(b, MODE_FINAL_RK2, e.ctrl.ws[e.ctrl.parity ^ 1]);
+  } else {
+    StageArgs<T> t = b;
+    t.ih = fld<T>(e.st[oth][0], e.tg[oth][0]);
+    t.ihu = fld<T>(e.st[oth][1], e.tg[oth][1]);
+    t.ihv = fld<T>(e.st[oth][2], e.tg[oth][2]);
+    t.oh = fldw<T>(e.st[cur][0], e.tg[cur][0]);
+    t.ohu = fldw<T>(e.st[cur][1], e.tg[cur][1]);
+    t.ohv = fldw<T>(e.st[cur][2], e.tg[cur][2]);
+    tail<T>(t, e.ctrl.ws[e.ctrl.parity ^ 1]);
+  }
+  Ctrl& c = e.ctrl;  // finalize_kernel
+  if (c.status != 0 || c.done != 0) return;
+  c.t = c.t + c.dt;
+  c.steps = c.steps + 1;
+  c.budget = c.budget - 1;
+  c.parity = c.parity ^ 1;
+}
+
+template <class T, int TW, int TH> int run_driver(Emu& e, bool use_target, double t_target, int64_t n) {
+  refresh<T, TW, TH>(e, e.with_bcs != 0);
+  Ctrl& c = e.ctrl;
+  c.status = 0;
+  c.done = 0;
+  c.use_target = use_target;
+  c.t_target = t_target;
+  c.dt_cap = __builtin_inf();
+  c.budget = use_target ? e.max_steps : n;
+  if (use_target) {
+    while (c.status == 0 && c.done == 0) driver_step<T, TW, TH>(e);
+  } else {
+    for (int64_t k = 0; k < n && c.status == 0; ++k) driver_step<T, TW, TH>(e);
+  }
+  return c.status;
+}
+
+// Tile shapes the emulator supports: the device shapes plus small odd ones
+// that put tile edges at every offset from the domain boundary.
+template <class T> struct Dispatch {
+  template <class F> static void with_shape(int tw, int th, F f);
+};
+
+#define HF_EMU_SHAPES(X) X(32, 16) X(16, 16) X(8, 4) X(4, 8) X(16, 8)
+
+template <class T, int TW, int TH> struct Ops {
+  static void ssprk2(Emu& e, double dt) { host_dt_step<T, TW, TH>(e, dt, true); }
+  static void euler(Emu& e, double dt) { host_dt_step<T, TW, TH>(e, dt, false); }
+  static int run(Emu& e, bool use_target, double tt, int64_t n) { return run_driver<T, TW, TH>(e, use_target, tt, n); }
+  static void ws(Emu& e, bool with_bcs) { refresh<T, TW, TH>(e, with_bcs); }
+  static void forcing(Emu& e, double dt, int fric, double tol, double rain, int nsrc) {
+    StageArgs<T> a = args<T>(e, e.cur, e.cur, e.cur, TW, TH);
+    a.dt_mode = 0;
+    a.dt_host = dt;
+    a.friction = fric;
+    a.friction_dry_tol = tol;
+    a.rain_rate = rain;
+    a.n_src = nsrc;
+    a.wave_speed = 0;
+    tail<T>(a, nullptr);
+  }
+};
+
+template <class T> int call(Emu& e, int op, double x, int64_t n, bool flag) {
+#define HF_CASE(W, H)                                   \
+  if (e.tw == W && e.th == H) {                         \
+    switch (op) {                                       \
+      case 0: Ops<T, W, H>::ssprk2(e, x); return 0;     \
+      case 1: Ops<T, W, H>::euler(e, x); return 0;      \
+      case 2: return Ops<T, W, H>::run(e, flag, x, n);  \
+      case 3: Ops<T, W, H>::ws(e, flag); return 0;      \
+      case 4: Ops<T, W, H>::forcing(e, x, 1, e.tmp_tol, 0.0, 0); return 0; \
+      case 5: Ops<T, W, H>::forcing(e, x, 0, 0.0, e.tmp_rain, 0); return 0; \
+      case 6: Ops<T, W, H>::forcing(e, x, 0, 0.0, 0.0, (int)e.srcs.size()); return 0; \
+    }                                                   \
+  }
+  HF_EMU_SHAPES(HF_CASE)
+#undef HF_CASE
+  std::fprintf(stderr, "emu: unsupported tile %dx%d\n", e.tw, e.th);
+  std::abort();
+}
+
+int dispatch(Emu& e, int op, double x, int64_t n, bool flag) {
+  return e.dual_kind ? call<dual>(e, op, x, n, flag) : call<double>(e, op, x, n, flag);
+}
+
+}  // namespace
+
+extern "C" {
+
+void* emu_create(int64_t nr, int64_t nc, double dx, double dy, int dual_kind, int tw, int th) {
+  Emu* e = new Emu();
+  e->nr = nr;
+  e->nc = nc;
+  e->dx = dx;
+  e->dy = dy;
+  e->dual_kind = dual_kind;
+  e->tw = tw;
+  e->th = th;
+  const size_t n = (size_t)(nr * nc);
+  for (int b = 0; b < 2; ++b)
+    for (int f = 0; f < 3; ++f) {
+      e->st[b][f].assign(n, 0.0);
+      if (dual_kind) e->tg[b][f].assign(n, 0.0);
+    }
+  e->z.assign(n, 0.0);
+  e->man.assign(n, 0.0);
+  if (dual_kind) {
+    e->zt.assign(n, 0.0);
+    e->mant.assign(n, 0.0);
+  }
+  e->ctrl.dt_cap = __builtin_inf();
+  return e;
+}
+
+void emu_destroy(void* p) { delete static_cast<Emu*>(p); }
+
+void emu_set_mesh(void* p, const double* bed, const double* bt, const double* man, const double* mt) {
+  Emu* e = static_cast<Emu*>(p);
+  const size_t n = e->z.size();
+  for (size_t k = 0; k < n; ++k) {
+    e->z[k] = bed[k];
+    e->man[k] = man[k];
+  }
+  if (e->dual_kind) {
+    e->has_zt = bt != nullptr;
+    for (size_t k = 0; k < n; ++k) {
+      e->zt[k] = bt ? bt[k] : 0.0;
+      e->mant[k] = mt ? mt[k] : 0.0;
+    }
+  }
+}
+
+void emu_set_state(void* p, const double* aos, const double* tan) {
+  Emu* e = static_cast<Emu*>(p);
+  const size_t n = e->z.size();
+  for (int f = 0; f < 3; ++f)
+    for (size_t k = 0; k < n; ++k) {
+      e->st[e->cur][f][k] = aos[3 * k + f];
+      if (e->dual_kind) e->tg[e->cur][f][k] = tan ? tan[3 * k + f] : 0.0;
+    }
+}
+
+void emu_get_state(void* p, double* aos, double* tan) {
+  Emu* e = static_cast<Emu*>(p);
+  const size_t n = e->z.size();
+  for (int f = 0; f < 3; ++f)
+    for (size_t k = 0; k < n; ++k) {
+      aos[3 * k + f] = e->st[e->cur][f][k];
+      if (e->dual_kind && tan) tan[3 * k + f] = e->tg[e->cur][f][k];
+    }
+}
+
+void emu_set_boundaries(void* p, const int* kinds, const double* params) {
+  Emu* e = static_cast<Emu*>(p);
+  for (int k = 0; k < 4; ++k) {
+    e->bc_kind[k] = kinds[k];
+    e->bc_param[k] = params[k];
+  }
+}
+
+void emu_configure(void* p, double cfl, int integrator, int with_bcs, double max_dt, int friction, double fdt,
+                   int64_t max_steps, double rain, const int64_t* rows, const int64_t* cols, const double* q,
+                   int64_t k) {
+  Emu* e = static_cast<Emu*>(p);
+  e->cfl = cfl;
+  e->integrator = integrator;
+  e->with_bcs = with_bcs;
+  e->max_dt = max_dt;
+  e->friction = friction;
+  e->fdt = fdt;
+  e->max_steps = max_steps;
+  e->rain = rain;
+  e->srcs.clear();
+  for (int64_t s = 0; s < k; ++s) e->srcs.push_back(PointSrc{rows[s], cols[s], q[s]});
+}
+
+void emu_ssprk2_step(void* p, double dt) { dispatch(*static_cast<Emu*>(p), 0, dt, 0, false); }
+void emu_forward_euler_step(void* p, double dt) { dispatch(*static_cast<Emu*>(p), 1, dt, 0, false); }
+int emu_run_until(void* p, double t) { return dispatch(*static_cast<Emu*>(p), 2, t, 0, true); }
+int emu_run_steps(void* p, int64_t n) { return dispatch(*static_cast<Emu*>(p), 2, 0.0, n, false); }
+
+double emu_cfl_time_step(void* p, double cfl, int with_bcs) {
+  Emu* e = static_cast<Emu*>(p);
+  dispatch(*e, 3, 0.0, 0, with_bcs != 0);
+  double sx = hfm::as_f64(e->ctrl.ws[e->ctrl.parity][0]);
+  double sy = hfm::as_f64(e->ctrl.ws[e->ctrl.parity][1]);
+  double denom = sx / e->dx + sy / e->dy;
+  return denom > 0.0 ? cfl / denom : __builtin_inf();
+}
+
+void emu_manning_friction_step(void* p, double dt, double tol) {
+  Emu* e = static_cast<Emu*>(p);
+  e->tmp_tol = tol;
+  dispatch(*e, 4, dt, 0, false);
+}
+void emu_apply_rain(void* p, double rate, double dt) {
+  Emu* e = static_cast<Emu*>(p);
+  if (rate == 0.0) return;
+  e->tmp_rain = rate;
+  dispatch(*e, 5, dt, 0, false);
+}
+// Point sources through the elementwise tail (each cell applies the sources
+// that hit it, in list order).
+void emu_apply_point_sources(void* p, const int64_t* rows, const int64_t* cols, const double* q, int64_t k,
+                             double dt) {
+  Emu* e = static_cast<Emu*>(p);
+  std::vector<PointSrc> keep = e->srcs;
+  e->srcs.clear();
+  for (int64_t s = 0; s < k; ++s) e->srcs.push_back(PointSrc{rows[s], cols[s], q[s]});
+  dispatch(*e, 6, dt, 0, false);
+  e->srcs = keep;
+}
+
+void emu_time(void* p, double* t, int64_t* steps) {
+  Emu* e = static_cast<Emu*>(p);
+  *t = e->ctrl.t;
+  *steps = e->ctrl.steps;
+}
+}

@@ -1,0 +1,99 @@
+"""The kernels' deterministic powers, and the C ABI surface (CPU only)."""
+import ctypes
+import os
+import re
+from decimal import Decimal, getcontext
+
+import numpy as np
+import pytest
+
+from oracle.pyoracle import lib as olib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _exact_pow(x, e):
+    getcontext().prec = 60
+    return (Decimal(e) * Decimal(x).ln()).exp()
+
+
+def test_pow_routines_are_correctly_rounded_on_random_inputs():
+    # hf_pow.h claims ~correct rounding for pow(x, 4.0/3.0) and pow(x, 4.0/3.0 - 1.0);
+    # check against 60-digit decimal arithmetic on log-uniform inputs.
+    rng = np.random.default_rng(11)
+    xs = 10.0 ** rng.uniform(-12, 6, 400)
+    L = olib()
+    for e, fn in ((4.0 / 3.0, L.hfo_pow_4_3), (4.0 / 3.0 - 1.0, L.hfo_pow_1_3m)):
+        for x in xs:
+            got = fn(float(x))
+            ex = _exact_pow(float(x), e)
+            ulp = np.spacing(got)
+            assert abs(Decimal(got) - ex) <= Decimal(ulp) * Decimal("0.5000001"), (x, e)
+
+
+def test_pow_routines_agree_with_glibc_within_one_ulp():
+    rng = np.random.default_rng(5)
+    xs = 10.0 ** rng.uniform(-9, 4, 20000)
+    L = olib()
+    for e, fn in ((4.0 / 3.0, L.hfo_pow_4_3), (4.0 / 3.0 - 1.0, L.hfo_pow_1_3m)):
+        a = np.array([fn(float(x)) for x in xs])
+        b = np.array([L.hfo_libm_pow(float(x), e) for x in xs])
+        d = np.abs(a - b) / np.spacing(b)
+        assert d.max() <= 1.0
+        assert np.mean(a != b) < 5e-3  # glibc 2.35 pow misses correct rounding rarely
+
+
+def test_pow_special_values():
+    L = olib()
+    assert L.hfo_pow_4_3(0.0) == 0.0 and L.hfo_pow_1_3m(0.0) == 0.0
+    assert L.hfo_pow_4_3(8.0) == pow(8.0, 4.0 / 3.0)
+    assert L.hfo_pow_4_3(float("inf")) == float("inf")
+    assert np.isnan(L.hfo_pow_4_3(-1.0))
+    assert L.hfo_pow_1_3m(5e-324) == pow(5e-324, 4.0 / 3.0 - 1.0)
+
+
+def _declared_symbols():
+    txt = open(os.path.join(ROOT, "include", "hydroflux_amd.h")).read()
+    return sorted(set(re.findall(r"\b(hf2d_[a-z_0-9]+)\s*\(", txt)))
+
+
+def test_library_exports_every_declared_symbol():
+    from hydroflux_amd import _lib
+
+    names = _declared_symbols()
+    assert len(names) >= 20
+    L = ctypes.CDLL(_lib.LIB_PATH)  # loading needs no GPU
+    for n in names:
+        assert hasattr(L, n), n
+    assert set(names) == set(_lib.SIGNATURES), set(names) ^ set(_lib.SIGNATURES)
+
+
+def test_library_is_gfx950_code_object():
+    from hydroflux_amd import _lib
+
+    blob = open(_lib.LIB_PATH, "rb").read()
+    assert b"gfx950" in blob
+
+
+def test_no_cpu_fallback_when_library_missing(monkeypatch, tmp_path):
+    from hydroflux_amd import _lib
+
+    monkeypatch.setattr(_lib, "LIB_PATH", str(tmp_path / "missing.so"))
+    monkeypatch.setattr(_lib, "_LIB", None)
+    with pytest.raises(ImportError):
+        _lib.lib()
+
+
+def test_mesh_validation_mirrors_reference_panics():
+    from hydroflux_amd import InvalidConfig, Mesh2D, ShapeMismatch
+
+    with pytest.raises(InvalidConfig):
+        Mesh2D(np.zeros((2, 2)), 0.0, 1.0, 0.03)
+    with pytest.raises(InvalidConfig):
+        Mesh2D(np.zeros((2, 2)), 1.0, -1.0, 0.03)
+    with pytest.raises(InvalidConfig):
+        Mesh2D(np.zeros((2, 2)), 1.0, 1.0, -0.01)
+    with pytest.raises(ShapeMismatch):
+        Mesh2D(np.zeros((2, 2)), 1.0, 1.0, np.zeros((3, 2)))
+    with pytest.raises(InvalidConfig):
+        Mesh2D(np.zeros((0, 2)), 1.0, 1.0, 0.0)
