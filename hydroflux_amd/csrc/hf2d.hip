@@ -29,8 +29,17 @@ namespace {
 
 // ------------------------------------------------------------ tile shapes
 template <class T> struct Shape;
+#ifndef HF_F64_TILE_W
+#define HF_F64_TILE_W 32
+#endif
+#ifndef HF_F64_TILE_H
+#define HF_F64_TILE_H 16
+#endif
+#ifndef HF_F64_THREADS
+#define HF_F64_THREADS 256
+#endif
 template <> struct Shape<double> {
-  static constexpr int TW = 32, TH = 16, NT = 256;
+  static constexpr int TW = HF_F64_TILE_W, TH = HF_F64_TILE_H, NT = HF_F64_THREADS;
 };
 template <> struct Shape<dual> {
   static constexpr int TW = 16, TH = 16, NT = 256;
