@@ -36,8 +36,26 @@ template <class T> struct Shape;
 #define HF_F64_TILE_H 16
 #endif
 #ifndef HF_F64_THREADS
-#define HF_F64_THREADS 256
+#define HF_F64_THREADS 1024
 #endif
+#ifndef HF_XCD_REMAP
+#define HF_XCD_REMAP 1
+#endif
+
+// Workgroups are dispatched round-robin over the 8 XCDs (workgroup b lands on
+// XCD b % 8), each with its own L2. Renumber tiles so every XCD walks one
+// contiguous band of tiles: horizontally adjacent tiles, which share halo
+// columns, then run on the same XCD at about the same time and the halo is
+// an L2 hit instead of a second HBM fetch.
+__device__ inline unsigned tile_of_block(unsigned b, unsigned nb) {
+#if HF_XCD_REMAP
+  const unsigned q = nb / 8u, r = nb % 8u, x = b % 8u;
+  return x * q + (x < r ? x : r) + b / 8u;
+#else
+  (void)nb;
+  return b;
+#endif
+}
 template <> struct Shape<double> {
   static constexpr int TW = HF_F64_TILE_W, TH = HF_F64_TILE_H, NT = HF_F64_THREADS;
 };
@@ -76,12 +94,13 @@ __global__ __launch_bounds__(Shape<T>::NT) void stage_kernel(StageArgs<T> a) {
   __shared__ typename St::S s;
   double dt;
   if (!hf::stage_prologue(a, blockIdx.x == 0 && threadIdx.x == 0, &dt)) return;
-  const int64_t ty = blockIdx.x / a.tiles_x, tx = blockIdx.x % a.tiles_x;
+  const unsigned tile = tile_of_block(blockIdx.x, gridDim.x);
+  const int64_t ty = tile / a.tiles_x, tx = tile % a.tiles_x;
   const int64_t gi0 = ty * TH, gj0 = tx * TW;
   const int tid = threadIdx.x;
   St::phase_load(s, a, gi0, gj0, tid, NT);
   __syncthreads();
-  St::phase_slopes(s, a, gi0, gj0, tid, NT);
+  St::phase_diffs(s, a, gi0, gj0, tid, NT);
   __syncthreads();
   St::phase_faces(s, a, gi0, gj0, tid, NT);
   __syncthreads();

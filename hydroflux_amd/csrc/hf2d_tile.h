@@ -16,8 +16,8 @@
 // four raw faces, whose MUSCL slopes read primitives one cell further out,
 // and a slope is zeroed if any cell within +-2 along its axis is dry
 // (update.rs:433-444). Hence: state (h, z, u, v) on a 3-cell halo, dry flags
-// on a 4-cell halo, slopes on a 2-cell halo (along their axis; 1 across),
-// faces and alpha on a 1-cell ring.
+// on a 4-cell halo, slopes on a 2-cell halo (along their axis; 1 across;
+// computed from per-face differences), faces and alpha on a 1-cell ring.
 //
 // Local coordinates: tile origin (gi0, gj0); li = gi - gi0 in [0, TH),
 // lj = gj - gj0 in [0, TW). x-face fj sits between local cells fj-1 and fj;
@@ -284,19 +284,24 @@ HF_HD void well_balanced_face(T lh, T lqn, T lqt, T rh, T rqn, T rqt, T zf, T* m
 template <class T, int TW, int TH> struct TileSmem {
   static constexpr int W3 = TW + 6, H3 = TH + 6, N3 = W3 * H3;
   static constexpr int W4 = TW + 8, H4 = TH + 8, N4 = W4 * H4;
-  static constexpr int WSX = TW + 4, HSX = TH + 2, NSX = WSX * HSX;
-  static constexpr int WSY = TW + 2, HSY = TH + 4, NSY = WSY * HSY;
+  // x-differences (prim_R - prim_L)/dx on x-faces fj in [-2, TW+2], rows [-1, TH]
+  static constexpr int WDX = TW + 5, HDX = TH + 2, NDX = WDX * HDX;
+  // y-differences on y-faces fi in [-2, TH+2], columns [-1, TW]
+  static constexpr int WDY = TW + 2, HDY = TH + 5, NDY = WDY * HDY;
+  // near-dry flags on rows [-2, TH+1] x columns [-2, TW+1]
+  static constexpr int WND = TW + 4, HND = TH + 4, NND = WND * HND;
   static constexpr int WFX = TW + 3, HFX = TH + 2, NFX = WFX * HFX;
   static constexpr int WFY = TW + 2, HFY = TH + 3, NFY = WFY * HFY;
   static constexpr int WA = TW + 2, HA = TH + 2, NA = WA * HA;
 
   T h[N3], z[N3], u[N3], v[N3];
-  T sxe[NSX], sxu[NSX], sxv[NSX];
-  T sye[NSY], syu[NSY], syv[NSY];
+  T dxe[NDX], dxu[NDX], dxv[NDX];
+  T dye[NDY], dyu[NDY], dyv[NDY];
   T fxm[NFX], fxa[NFX], fxb[NFX], fxt[NFX];
   T fym[NFY], fya[NFY], fyb[NFY], fyt[NFY];
   T alpha[NA];
   unsigned char dry[N4];
+  unsigned char nd[N3];  // bit 0: a dry cell within +-2 along x; bit 1: along y
   double red[2][16];
 
   static HF_HD int i3(int li, int lj) {
@@ -307,13 +312,13 @@ template <class T, int TW, int TH> struct TileSmem {
     HF_BOUNDS_CHECK(li >= -4 && li < TH + 4 && lj >= -4 && lj < TW + 4);
     return (li + 4) * W4 + (lj + 4);
   }
-  static HF_HD int isx(int li, int lj) {
-    HF_BOUNDS_CHECK(li >= -1 && li < TH + 1 && lj >= -2 && lj < TW + 2);
-    return (li + 1) * WSX + (lj + 2);
+  static HF_HD int idx_(int li, int fj) {
+    HF_BOUNDS_CHECK(li >= -1 && li < TH + 1 && fj >= -2 && fj < TW + 3);
+    return (li + 1) * WDX + (fj + 2);
   }
-  static HF_HD int isy(int li, int lj) {
-    HF_BOUNDS_CHECK(li >= -2 && li < TH + 2 && lj >= -1 && lj < TW + 1);
-    return (li + 2) * WSY + (lj + 1);
+  static HF_HD int idy_(int fi, int lj) {
+    HF_BOUNDS_CHECK(fi >= -2 && fi < TH + 3 && lj >= -1 && lj < TW + 1);
+    return (fi + 2) * WDY + (lj + 1);
   }
   static HF_HD int ifx(int li, int fj) {
     HF_BOUNDS_CHECK(li >= -1 && li < TH + 1 && fj >= -1 && fj < TW + 2);
@@ -406,74 +411,118 @@ template <class T, int TW, int TH> struct Stage {
     }
   }
 
-  // Phase 2: minmod slopes of (eta, u, v) in x and y (update.rs:459-540).
-  static HF_HD void phase_slopes(S& s, const StageArgs<T>& a, int64_t gi0, int64_t gj0, int tid, int nt) {
+  // Phase 2: per-face differences of the primitives, (R - L)/dx and
+  // (B - T)/dy, shared by the two cells whose minmod slopes use them
+  // (update.rs:459-540 evaluates each twice; the values are identical), and
+  // the near-dry flags of update.rs:433-444 (window clipped to the domain).
+  static HF_HD void phase_diffs(S& s, const StageArgs<T>& a, int64_t gi0, int64_t gj0, int tid, int nt) {
     const T Z = cst<T>(0.0);
     const double dx = a.dx, dy = a.dy;
-    for (int k = tid; k < S::NSX; k += nt) {
-      const int li = k / S::WSX - 1, lj = k % S::WSX - 2;
-      const int64_t gi = gi0 + li, gj = gj0 + lj;
-      T se = Z, su = Z, sv = Z;
-      if (gi >= 0 && gi < a.nr && gj >= 0 && gj < a.nc && a.nc >= 2) {
-        bool near_dry = false;
-        for (int d = -2; d <= 2; ++d) near_dry = near_dry || (s.dry[S::i4(li, lj + d)] != 0);
-        if (!near_dry) {
-          const int c = S::i3(li, lj);
-          if (gj == 0) {
-            const int r = S::i3(li, lj + 1);
-            se = (s.h[r] + s.z[r] - (s.h[c] + s.z[c])) / dx;
-            su = (s.u[r] - s.u[c]) / dx;
-            sv = (s.v[r] - s.v[c]) / dx;
-          } else if (gj + 1 == a.nc) {
-            const int l = S::i3(li, lj - 1);
-            se = (s.h[c] + s.z[c] - (s.h[l] + s.z[l])) / dx;
-            su = (s.u[c] - s.u[l]) / dx;
-            sv = (s.v[c] - s.v[l]) / dx;
-          } else {
-            const int l = S::i3(li, lj - 1), r = S::i3(li, lj + 1);
-            T el = s.h[l] + s.z[l], ec = s.h[c] + s.z[c], er = s.h[r] + s.z[r];
-            se = minmod((ec - el) / dx, (er - ec) / dx);
-            su = minmod((s.u[c] - s.u[l]) / dx, (s.u[r] - s.u[c]) / dx);
-            sv = minmod((s.v[c] - s.v[l]) / dx, (s.v[r] - s.v[c]) / dx);
-          }
+    const int n_total = S::NDX + S::NDY + S::NND;
+    for (int k = tid; k < n_total; k += nt) {
+      if (k < S::NDX) {
+        const int li = k / S::WDX - 1, fj = k % S::WDX - 2;
+        const int64_t gi = gi0 + li, gj = gj0 + fj;
+        T de = Z, du = Z, dv = Z;
+        if (gi >= 0 && gi < a.nr && gj >= 1 && gj < a.nc) {
+          const int l = S::i3(li, fj - 1), r = S::i3(li, fj);
+          de = (s.h[r] + s.z[r] - (s.h[l] + s.z[l])) / dx;
+          du = (s.u[r] - s.u[l]) / dx;
+          dv = (s.v[r] - s.v[l]) / dx;
         }
-      }
-      s.sxe[k] = se;
-      s.sxu[k] = su;
-      s.sxv[k] = sv;
-    }
-    for (int k = tid; k < S::NSY; k += nt) {
-      const int li = k / S::WSY - 2, lj = k % S::WSY - 1;
-      const int64_t gi = gi0 + li, gj = gj0 + lj;
-      T se = Z, su = Z, sv = Z;
-      if (gi >= 0 && gi < a.nr && gj >= 0 && gj < a.nc && a.nr >= 2) {
-        bool near_dry = false;
-        for (int d = -2; d <= 2; ++d) near_dry = near_dry || (s.dry[S::i4(li + d, lj)] != 0);
-        if (!near_dry) {
-          const int c = S::i3(li, lj);
-          if (gi == 0) {
-            const int b = S::i3(li + 1, lj);
-            se = (s.h[b] + s.z[b] - (s.h[c] + s.z[c])) / dy;
-            su = (s.u[b] - s.u[c]) / dy;
-            sv = (s.v[b] - s.v[c]) / dy;
-          } else if (gi + 1 == a.nr) {
-            const int t = S::i3(li - 1, lj);
-            se = (s.h[c] + s.z[c] - (s.h[t] + s.z[t])) / dy;
-            su = (s.u[c] - s.u[t]) / dy;
-            sv = (s.v[c] - s.v[t]) / dy;
-          } else {
-            const int t = S::i3(li - 1, lj), b = S::i3(li + 1, lj);
-            T et = s.h[t] + s.z[t], ec = s.h[c] + s.z[c], eb = s.h[b] + s.z[b];
-            se = minmod((ec - et) / dy, (eb - ec) / dy);
-            su = minmod((s.u[c] - s.u[t]) / dy, (s.u[b] - s.u[c]) / dy);
-            sv = minmod((s.v[c] - s.v[t]) / dy, (s.v[b] - s.v[c]) / dy);
-          }
+        s.dxe[k] = de;
+        s.dxu[k] = du;
+        s.dxv[k] = dv;
+      } else if (k < S::NDX + S::NDY) {
+        const int kk = k - S::NDX;
+        const int fi = kk / S::WDY - 2, lj = kk % S::WDY - 1;
+        const int64_t gi = gi0 + fi, gj = gj0 + lj;
+        T de = Z, du = Z, dv = Z;
+        if (gj >= 0 && gj < a.nc && gi >= 1 && gi < a.nr) {
+          const int t = S::i3(fi - 1, lj), b = S::i3(fi, lj);
+          de = (s.h[b] + s.z[b] - (s.h[t] + s.z[t])) / dy;
+          du = (s.u[b] - s.u[t]) / dy;
+          dv = (s.v[b] - s.v[t]) / dy;
         }
+        s.dye[kk] = de;
+        s.dyu[kk] = du;
+        s.dyv[kk] = dv;
+      } else {
+        const int kk = k - S::NDX - S::NDY;
+        const int li = kk / S::WND - 2, lj = kk % S::WND - 2;
+        bool nx = false, ny = false;
+        for (int d = -2; d <= 2; ++d) {
+          nx = nx || (s.dry[S::i4(li, lj + d)] != 0);
+          ny = ny || (s.dry[S::i4(li + d, lj)] != 0);
+        }
+        s.nd[S::i3(li, lj)] = (unsigned char)((nx ? 1 : 0) | (ny ? 2 : 0));
       }
-      s.sye[k] = se;
-      s.syu[k] = su;
-      s.syv[k] = sv;
     }
+  }
+
+  // Minmod slope of cell (li, lj) along x from the face differences
+  // (update.rs:459-499): zero near a dry cell or on a 1-column mesh,
+  // one-sided on the first/last column, else minmod(left, right).
+  static HF_HD void slope_x(const S& s, const StageArgs<T>& a, int li, int lj, int64_t gj, T* se, T* su,
+                            T* sv) {
+    const T Z = cst<T>(0.0);
+    if (a.nc < 2 || (s.nd[S::i3(li, lj)] & 1)) {
+      *se = Z;
+      *su = Z;
+      *sv = Z;
+    } else if (gj == 0) {
+      const int r = S::idx_(li, lj + 1);
+      *se = s.dxe[r];
+      *su = s.dxu[r];
+      *sv = s.dxv[r];
+    } else if (gj + 1 == a.nc) {
+      const int l = S::idx_(li, lj);
+      *se = s.dxe[l];
+      *su = s.dxu[l];
+      *sv = s.dxv[l];
+    } else {
+      const int l = S::idx_(li, lj), r = S::idx_(li, lj + 1);
+      *se = minmod(s.dxe[l], s.dxe[r]);
+      *su = minmod(s.dxu[l], s.dxu[r]);
+      *sv = minmod(s.dxv[l], s.dxv[r]);
+    }
+  }
+  static HF_HD T slope_x_eta(const S& s, const StageArgs<T>& a, int li, int lj, int64_t gj) {
+    if (a.nc < 2 || (s.nd[S::i3(li, lj)] & 1)) return cst<T>(0.0);
+    if (gj == 0) return s.dxe[S::idx_(li, lj + 1)];
+    if (gj + 1 == a.nc) return s.dxe[S::idx_(li, lj)];
+    return minmod(s.dxe[S::idx_(li, lj)], s.dxe[S::idx_(li, lj + 1)]);
+  }
+  // update.rs:503-540
+  static HF_HD void slope_y(const S& s, const StageArgs<T>& a, int li, int lj, int64_t gi, T* se, T* su,
+                            T* sv) {
+    const T Z = cst<T>(0.0);
+    if (a.nr < 2 || (s.nd[S::i3(li, lj)] & 2)) {
+      *se = Z;
+      *su = Z;
+      *sv = Z;
+    } else if (gi == 0) {
+      const int b = S::idy_(li + 1, lj);
+      *se = s.dye[b];
+      *su = s.dyu[b];
+      *sv = s.dyv[b];
+    } else if (gi + 1 == a.nr) {
+      const int t = S::idy_(li, lj);
+      *se = s.dye[t];
+      *su = s.dyu[t];
+      *sv = s.dyv[t];
+    } else {
+      const int t = S::idy_(li, lj), b = S::idy_(li + 1, lj);
+      *se = minmod(s.dye[t], s.dye[b]);
+      *su = minmod(s.dyu[t], s.dyu[b]);
+      *sv = minmod(s.dyv[t], s.dyv[b]);
+    }
+  }
+  static HF_HD T slope_y_eta(const S& s, const StageArgs<T>& a, int li, int lj, int64_t gi) {
+    if (a.nr < 2 || (s.nd[S::i3(li, lj)] & 2)) return cst<T>(0.0);
+    if (gi == 0) return s.dye[S::idy_(li + 1, lj)];
+    if (gi + 1 == a.nr) return s.dye[S::idy_(li, lj)];
+    return minmod(s.dye[S::idy_(li, lj)], s.dye[S::idy_(li + 1, lj)]);
   }
 
   // Raw state of a domain cell (boundary faces use it unreconstructed,
@@ -486,93 +535,110 @@ template <class T, int TW, int TH> struct Stage {
     *hv = a.ihv.ld(idx);
   }
 
-  // Phase 3: every x- and y-face of the tile and its 1-cell ring
-  // (update.rs:560-623, 890-922).
-  static HF_HD void phase_faces(S& s, const StageArgs<T>& a, int64_t gi0, int64_t gj0, int tid, int nt) {
+  // One x-face (local row li, face column fj) into the face arrays
+  // (update.rs:560-588, 890-905).
+  static HF_HD void face_x(S& s, const StageArgs<T>& a, int64_t gi0, int64_t gj0, int k) {
     const T Z = cst<T>(0.0);
-    const double half_dx = 0.5 * a.dx, half_dy = 0.5 * a.dy;
-    for (int k = tid; k < S::NFX; k += nt) {
-      const int li = k / S::WFX - 1, fj = k % S::WFX - 1;
-      const int64_t gi = gi0 + li, gjf = gj0 + fj;
-      T m = Z, na = Z, nb = Z, tg = Z;
-      if (gi >= 0 && gi < a.nr && gjf >= 0 && gjf <= a.nc) {
-        T zf = zface_x(s, a, gi0, gj0, li, fj);
-        if (gjf == 0 || gjf == a.nc) {
-          const bool west = (gjf == 0);
-          const int lc = west ? fj : fj - 1;
-          const int ln = west ? fj + 1 : fj - 2;
-          const int64_t gc = west ? 0 : a.nc - 1;
-          T h, hu, hv, gh, ghu, ghv;
-          raw(s, a, li, lc, gi, gc, &h, &hu, &hv);
-          T zc = s.z[S::i3(li, lc)];
-          T zn = (a.nc >= 2) ? s.z[S::i3(li, ln)] : zc;
-          const int side = west ? SIDE_W : SIDE_E;
-          ghost_state(a.bc_kind[side], a.bc_param[side], side, h, hu, hv, zc, zn, a.nc >= 2, a.dx,
-                      a.man.ld(gi * a.nc + gc), &gh, &ghu, &ghv);
-          if (west)
-            well_balanced_face(gh, ghu, ghv, h, hu, hv, zf, &m, &na, &nb, &tg);
-          else
-            well_balanced_face(h, hu, hv, gh, ghu, ghv, zf, &m, &na, &nb, &tg);
-        } else if (!(s.dry[S::i4(li, fj - 1)] && s.dry[S::i4(li, fj)])) {
-          const int l = S::i3(li, fj - 1), r = S::i3(li, fj);
-          const int sl = S::isx(li, fj - 1), sr = S::isx(li, fj);
-          T eta_m = s.h[l] + s.z[l] + s.sxe[sl] * half_dx;
-          T u_m = s.u[l] + s.sxu[sl] * half_dx;
-          T v_m = s.v[l] + s.sxv[sl] * half_dx;
-          T eta_p = s.h[r] + s.z[r] - s.sxe[sr] * half_dx;
-          T u_p = s.u[r] - s.sxu[sr] * half_dx;
-          T v_p = s.v[r] - s.sxv[sr] * half_dx;
-          T h_m = rmax(eta_m - zf, Z);
-          T h_p = rmax(eta_p - zf, Z);
-          well_balanced_face(h_m, h_m * u_m, h_m * v_m, h_p, h_p * u_p, h_p * v_p, zf, &m, &na, &nb, &tg);
-        }
+    const int li = k / S::WFX - 1, fj = k % S::WFX - 1;
+    const int64_t gi = gi0 + li, gjf = gj0 + fj;
+    T m = Z, na = Z, nb = Z, tg = Z;
+    if (gi >= 0 && gi < a.nr && gjf >= 0 && gjf <= a.nc) {
+      T zf = zface_x(s, a, gi0, gj0, li, fj);
+      if (gjf == 0 || gjf == a.nc) {
+        const bool west = (gjf == 0);
+        const int lc = west ? fj : fj - 1;
+        const int ln = west ? fj + 1 : fj - 2;
+        const int64_t gc = west ? 0 : a.nc - 1;
+        T h, hu, hv, gh, ghu, ghv;
+        raw(s, a, li, lc, gi, gc, &h, &hu, &hv);
+        T zc = s.z[S::i3(li, lc)];
+        T zn = (a.nc >= 2) ? s.z[S::i3(li, ln)] : zc;
+        const int side = west ? SIDE_W : SIDE_E;
+        ghost_state(a.bc_kind[side], a.bc_param[side], side, h, hu, hv, zc, zn, a.nc >= 2, a.dx,
+                    a.man.ld(gi * a.nc + gc), &gh, &ghu, &ghv);
+        if (west)
+          well_balanced_face(gh, ghu, ghv, h, hu, hv, zf, &m, &na, &nb, &tg);
+        else
+          well_balanced_face(h, hu, hv, gh, ghu, ghv, zf, &m, &na, &nb, &tg);
+      } else if (!(s.dry[S::i4(li, fj - 1)] && s.dry[S::i4(li, fj)])) {
+        const double half_dx = 0.5 * a.dx;
+        const int l = S::i3(li, fj - 1), r = S::i3(li, fj);
+        T sle, slu, slv, sre, sru, srv;
+        slope_x(s, a, li, fj - 1, gjf - 1, &sle, &slu, &slv);
+        slope_x(s, a, li, fj, gjf, &sre, &sru, &srv);
+        T eta_m = s.h[l] + s.z[l] + sle * half_dx;
+        T u_m = s.u[l] + slu * half_dx;
+        T v_m = s.v[l] + slv * half_dx;
+        T eta_p = s.h[r] + s.z[r] - sre * half_dx;
+        T u_p = s.u[r] - sru * half_dx;
+        T v_p = s.v[r] - srv * half_dx;
+        T h_m = rmax(eta_m - zf, Z);
+        T h_p = rmax(eta_p - zf, Z);
+        well_balanced_face(h_m, h_m * u_m, h_m * v_m, h_p, h_p * u_p, h_p * v_p, zf, &m, &na, &nb, &tg);
       }
-      s.fxm[k] = m;
-      s.fxa[k] = na;
-      s.fxb[k] = nb;
-      s.fxt[k] = tg;
     }
-    for (int k = tid; k < S::NFY; k += nt) {
-      const int fi = k / S::WFY - 1, lj = k % S::WFY - 1;
-      const int64_t gif = gi0 + fi, gj = gj0 + lj;
-      T m = Z, na = Z, nb = Z, tg = Z;
-      if (gj >= 0 && gj < a.nc && gif >= 0 && gif <= a.nr) {
-        T zf = zface_y(s, a, gi0, gj0, fi, lj);
-        if (gif == 0 || gif == a.nr) {
-          const bool north = (gif == 0);
-          const int lc = north ? fi : fi - 1;
-          const int ln = north ? fi + 1 : fi - 2;
-          const int64_t gc = north ? 0 : a.nr - 1;
-          T h, hu, hv, gh, ghu, ghv;
-          raw(s, a, lc, lj, gc, gj, &h, &hu, &hv);
-          T zc = s.z[S::i3(lc, lj)];
-          T zn = (a.nr >= 2) ? s.z[S::i3(ln, lj)] : zc;
-          const int side = north ? SIDE_N : SIDE_S;
-          ghost_state(a.bc_kind[side], a.bc_param[side], side, h, hu, hv, zc, zn, a.nr >= 2, a.dy,
-                      a.man.ld(gc * a.nc + gj), &gh, &ghu, &ghv);
-          // y-face: normal momentum is hv, tangential hu (riemann.rs:97-107)
-          if (north)
-            well_balanced_face(gh, ghv, ghu, h, hv, hu, zf, &m, &na, &nb, &tg);
-          else
-            well_balanced_face(h, hv, hu, gh, ghv, ghu, zf, &m, &na, &nb, &tg);
-        } else if (!(s.dry[S::i4(fi - 1, lj)] && s.dry[S::i4(fi, lj)])) {
-          const int t = S::i3(fi - 1, lj), b = S::i3(fi, lj);
-          const int st = S::isy(fi - 1, lj), sb = S::isy(fi, lj);
-          T eta_m = s.h[t] + s.z[t] + s.sye[st] * half_dy;
-          T u_m = s.u[t] + s.syu[st] * half_dy;
-          T v_m = s.v[t] + s.syv[st] * half_dy;
-          T eta_p = s.h[b] + s.z[b] - s.sye[sb] * half_dy;
-          T u_p = s.u[b] - s.syu[sb] * half_dy;
-          T v_p = s.v[b] - s.syv[sb] * half_dy;
-          T h_m = rmax(eta_m - zf, Z);
-          T h_p = rmax(eta_p - zf, Z);
-          well_balanced_face(h_m, h_m * v_m, h_m * u_m, h_p, h_p * v_p, h_p * u_p, zf, &m, &na, &nb, &tg);
-        }
+    s.fxm[k] = m;
+    s.fxa[k] = na;
+    s.fxb[k] = nb;
+    s.fxt[k] = tg;
+  }
+
+  // One y-face (face row fi, local column lj) (update.rs:595-623, 907-922).
+  static HF_HD void face_y(S& s, const StageArgs<T>& a, int64_t gi0, int64_t gj0, int k) {
+    const T Z = cst<T>(0.0);
+    const int fi = k / S::WFY - 1, lj = k % S::WFY - 1;
+    const int64_t gif = gi0 + fi, gj = gj0 + lj;
+    T m = Z, na = Z, nb = Z, tg = Z;
+    if (gj >= 0 && gj < a.nc && gif >= 0 && gif <= a.nr) {
+      T zf = zface_y(s, a, gi0, gj0, fi, lj);
+      if (gif == 0 || gif == a.nr) {
+        const bool north = (gif == 0);
+        const int lc = north ? fi : fi - 1;
+        const int ln = north ? fi + 1 : fi - 2;
+        const int64_t gc = north ? 0 : a.nr - 1;
+        T h, hu, hv, gh, ghu, ghv;
+        raw(s, a, lc, lj, gc, gj, &h, &hu, &hv);
+        T zc = s.z[S::i3(lc, lj)];
+        T zn = (a.nr >= 2) ? s.z[S::i3(ln, lj)] : zc;
+        const int side = north ? SIDE_N : SIDE_S;
+        ghost_state(a.bc_kind[side], a.bc_param[side], side, h, hu, hv, zc, zn, a.nr >= 2, a.dy,
+                    a.man.ld(gc * a.nc + gj), &gh, &ghu, &ghv);
+        // y-face: normal momentum is hv, tangential hu (riemann.rs:97-107)
+        if (north)
+          well_balanced_face(gh, ghv, ghu, h, hv, hu, zf, &m, &na, &nb, &tg);
+        else
+          well_balanced_face(h, hv, hu, gh, ghv, ghu, zf, &m, &na, &nb, &tg);
+      } else if (!(s.dry[S::i4(fi - 1, lj)] && s.dry[S::i4(fi, lj)])) {
+        const double half_dy = 0.5 * a.dy;
+        const int t = S::i3(fi - 1, lj), b = S::i3(fi, lj);
+        T ste, stu, stv, sbe, sbu, sbv;
+        slope_y(s, a, fi - 1, lj, gif - 1, &ste, &stu, &stv);
+        slope_y(s, a, fi, lj, gif, &sbe, &sbu, &sbv);
+        T eta_m = s.h[t] + s.z[t] + ste * half_dy;
+        T u_m = s.u[t] + stu * half_dy;
+        T v_m = s.v[t] + stv * half_dy;
+        T eta_p = s.h[b] + s.z[b] - sbe * half_dy;
+        T u_p = s.u[b] - sbu * half_dy;
+        T v_p = s.v[b] - sbv * half_dy;
+        T h_m = rmax(eta_m - zf, Z);
+        T h_p = rmax(eta_p - zf, Z);
+        well_balanced_face(h_m, h_m * v_m, h_m * u_m, h_p, h_p * v_p, h_p * u_p, zf, &m, &na, &nb, &tg);
       }
-      s.fym[k] = m;
-      s.fya[k] = na;
-      s.fyb[k] = nb;
-      s.fyt[k] = tg;
+    }
+    s.fym[k] = m;
+    s.fya[k] = na;
+    s.fyb[k] = nb;
+    s.fyt[k] = tg;
+  }
+
+  // Phase 3: every x- and y-face of the tile and its 1-cell ring, in one
+  // loop so that lanes stay busy across the two face sets.
+  static HF_HD void phase_faces(S& s, const StageArgs<T>& a, int64_t gi0, int64_t gj0, int tid, int nt) {
+    for (int k = tid; k < S::NFX + S::NFY; k += nt) {
+      if (k < S::NFX)
+        face_x(s, a, gi0, gj0, k);
+      else
+        face_y(s, a, gi0, gj0, k - S::NFX);
     }
   }
 
@@ -763,7 +829,7 @@ template <class T, int TW, int TH> struct Stage {
         T dhv = (rt - lt) * dt_dx + (ba - tb) * dt_dy;
 
         T eta_cell = h_old + s.z[c3];
-        T sex = s.sxe[S::isx(li, lj)];
+        T sex = slope_x_eta(s, a, li, lj, gj);
         T eta_left = (gj > 0) ? eta_cell - sex * (0.5 * a.dx) : eta_cell;
         T eta_right = (gj + 1 < a.nc) ? eta_cell + sex * (0.5 * a.dx) : eta_cell;
         T hfl = rmax(eta_left - zface_x(s, a, gi0, gj0, li, lj), Z);
@@ -772,7 +838,7 @@ template <class T, int TW, int TH> struct Stage {
         T zer = eta_right - hfr;
         T s_hu = (hfl + hfr) * (zel - zer) * (0.5 * GRAVITY) / a.dx;
 
-        T sey = s.sye[S::isy(li, lj)];
+        T sey = slope_y_eta(s, a, li, lj, gi);
         T eta_top = (gi > 0) ? eta_cell - sey * (0.5 * a.dy) : eta_cell;
         T eta_bot = (gi + 1 < a.nr) ? eta_cell + sey * (0.5 * a.dy) : eta_cell;
         T hft = rmax(eta_top - zface_y(s, a, gi0, gj0, li, lj), Z);

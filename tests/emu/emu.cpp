@@ -102,7 +102,7 @@ void launch(const StageArgs<T>& a, int mode, unsigned long long* dst) {
     if (!stage_prologue(a, b == 0, &dt)) return;
     const int64_t gi0 = (b / a.tiles_x) * TH, gj0 = (b % a.tiles_x) * TW;
     for (int t = 0; t < NT; ++t) St::phase_load(*s, a, gi0, gj0, t, NT);
-    for (int t = 0; t < NT; ++t) St::phase_slopes(*s, a, gi0, gj0, t, NT);
+    for (int t = 0; t < NT; ++t) St::phase_diffs(*s, a, gi0, gj0, t, NT);
     for (int t = 0; t < NT; ++t) St::phase_faces(*s, a, gi0, gj0, t, NT);
     for (int t = 0; t < NT; ++t) St::phase_alpha(*s, a, gi0, gj0, t, NT, dt);
     for (int t = 0; t < NT; ++t) {
@@ -234,7 +234,7 @@ template <class T> struct Dispatch {
   template <class F> static void with_shape(int tw, int th, F f);
 };
 
-#define HF_EMU_SHAPES(X) X(32, 16) X(16, 16) X(8, 4) X(4, 8) X(16, 8)
+#define HF_EMU_SHAPES(X) X(32, 16) X(16, 16) X(8, 4) X(4, 8) X(16, 8) X(24, 16)
 
 template <class T, int TW, int TH> struct Ops {
   static void ssprk2(Emu& e, double dt) { host_dt_step<T, TW, TH>(e, dt, true); }
