@@ -60,8 +60,28 @@ SIGNATURES = {
     "hf2d_synchronize": (C.c_int, [C.c_void_p]),
     "hf2d_set_kernel_timing": (C.c_int, [C.c_void_p, C.c_int32]),
     "hf2d_kernel_timing": (C.c_int, [C.c_void_p, _D, _I64]),
+    "hf2d_total_volume": (C.c_int, [C.c_void_p, _D]),
     "hf2d_copy_probe": (C.c_int, [C.c_int64, C.c_int32, _D]),
     "hf2d_math_probe": (C.c_int, [_D, _D, C.c_int64, _D]),
+    "hf2d_batch_create": (C.c_int, [C.c_int32, C.c_int64, C.c_int32, C.POINTER(C.c_void_p)]),
+    "hf2d_batch_destroy": (C.c_int, [C.c_void_p]),
+    "hf2d_batch_last_error": (C.c_char_p, [C.c_void_p]),
+    "hf2d_batch_footprint": (C.c_int, [C.c_int32, C.c_int64, C.c_int64, C.c_int64, _I64]),
+    "hf2d_batch_free_bytes": (C.c_int, [C.c_void_p, _I64, _I64]),
+    "hf2d_batch_add": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64, C.c_double, C.c_double, _D, _D, _D, _D, _D,
+                                 C.POINTER(C.c_int32), _D, C.POINTER(HfConfig), C.POINTER(HfPointSource),
+                                 C.c_int64, C.c_double, C.POINTER(C.c_int32)]),
+    "hf2d_batch_remove": (C.c_int, [C.c_void_p, C.c_int32]),
+    "hf2d_batch_step": (C.c_int, [C.c_void_p, C.c_int64, C.c_int32]),
+    "hf2d_batch_synchronize": (C.c_int, [C.c_void_p]),
+    "hf2d_batch_poll": (C.c_int, [C.c_void_p, C.c_int32, _D, _I64, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                  C.POINTER(C.c_int32)]),
+    "hf2d_batch_download": (C.c_int, [C.c_void_p, C.c_int32, _D, _D]),
+    "hf2d_batch_config_count": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32)]),
+    "hf2d_batch_config_info": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                         C.POINTER(C.c_int32)]),
+    "hf2d_batch_set_config": (C.c_int, [C.c_void_p, C.c_int32]),
+    "hf2d_batch_autotune": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_int32), _D]),
 }
 
 _LIB = None

@@ -2,7 +2,8 @@
 
     python -m hydroflux_amd.build
 
-compiles hydroflux_amd/csrc/hf2d.hip into hydroflux_amd/lib/libhf2d.so.
+compiles hydroflux_amd/csrc/hf2d.hip (single grids) and hf2d_batch.hip (batched
+scenarios) into hydroflux_amd/lib/libhf2d.so.
 -ffp-contract=off keeps every a*b+c unfused (the reference's Rust arithmetic
 never contracts), which is what makes the f64 step bit-identical to the CPU
 restatement; packed FP32 instructions are disabled.
@@ -36,7 +37,8 @@ def build_library() -> str:
     os.makedirs(out_dir, exist_ok=True)
     out = os.path.join(out_dir, "libhf2d.so")
     tmp = out + ".tmp"
-    _run([HIPCC, *HIP_FLAGS, "-o", tmp, os.path.join(HERE, "csrc", "hf2d.hip")])
+    _run([HIPCC, *HIP_FLAGS, "-o", tmp, os.path.join(HERE, "csrc", "hf2d.hip"),
+          os.path.join(HERE, "csrc", "hf2d_batch.hip")])
     os.replace(tmp, out)
     return out
 

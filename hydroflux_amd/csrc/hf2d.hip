@@ -16,194 +16,8 @@
 #include <vector>
 
 #include "../../include/hydroflux_amd.h"
-#include "hf2d_tile.h"
+#include "hf2d_device.h"
 
-using hf::Ctrl;
-using hf::dual;
-using hf::Fld;
-using hf::FldW;
-using hf::PointSrc;
-using hf::StageArgs;
-
-namespace {
-
-// ------------------------------------------------------------ tile shapes
-template <class T> struct Shape;
-#ifndef HF_F64_TILE_W
-#define HF_F64_TILE_W 32
-#endif
-#ifndef HF_F64_TILE_H
-#define HF_F64_TILE_H 16
-#endif
-#ifndef HF_F64_THREADS
-#define HF_F64_THREADS 1024
-#endif
-#ifndef HF_XCD_REMAP
-#define HF_XCD_REMAP 1
-#endif
-
-// Workgroups are dispatched round-robin over the 8 XCDs (workgroup b lands on
-// XCD b % 8), each with its own L2. Renumber tiles so every XCD walks one
-// contiguous band of tiles: horizontally adjacent tiles, which share halo
-// columns, then run on the same XCD at about the same time and the halo is
-// an L2 hit instead of a second HBM fetch.
-__device__ inline unsigned tile_of_block(unsigned b, unsigned nb) {
-#if HF_XCD_REMAP
-  const unsigned q = nb / 8u, r = nb % 8u, x = b % 8u;
-  return x * q + (x < r ? x : r) + b / 8u;
-#else
-  (void)nb;
-  return b;
-#endif
-}
-template <> struct Shape<double> {
-  static constexpr int TW = HF_F64_TILE_W, TH = HF_F64_TILE_H, NT = HF_F64_THREADS;
-};
-template <> struct Shape<dual> {
-  static constexpr int TW = 16, TH = 16, NT = 256;
-};
-
-__device__ inline void block_max_to(double sx, double sy, double (*red)[16], unsigned long long* dst) {
-  const int tid = threadIdx.x;
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    sx = __builtin_fmax(sx, __shfl_xor(sx, off));
-    sy = __builtin_fmax(sy, __shfl_xor(sy, off));
-  }
-  const int wave = tid >> 6, lane = tid & 63;
-  if (lane == 0) {
-    red[0][wave] = sx;
-    red[1][wave] = sy;
-  }
-  __syncthreads();
-  if (tid == 0) {
-    const int nw = (blockDim.x + 63) >> 6;
-    for (int w = 1; w < nw; ++w) {
-      sx = __builtin_fmax(sx, red[0][w]);
-      sy = __builtin_fmax(sy, red[1][w]);
-    }
-    atomicMax(dst + 0, hfm::as_u64(sx));
-    atomicMax(dst + 1, hfm::as_u64(sy));
-  }
-}
-
-template <class T, int MODE>
-__global__ __launch_bounds__(Shape<T>::NT) void stage_kernel(StageArgs<T> a) {
-  constexpr int TW = Shape<T>::TW, TH = Shape<T>::TH, NT = Shape<T>::NT;
-  using St = hf::Stage<T, TW, TH>;
-  __shared__ typename St::S s;
-  double dt;
-  if (!hf::stage_prologue(a, blockIdx.x == 0 && threadIdx.x == 0, &dt)) return;
-  const unsigned tile = tile_of_block(blockIdx.x, gridDim.x);
-  const int64_t ty = tile / a.tiles_x, tx = tile % a.tiles_x;
-  const int64_t gi0 = ty * TH, gj0 = tx * TW;
-  const int tid = threadIdx.x;
-  St::phase_load(s, a, gi0, gj0, tid, NT);
-  __syncthreads();
-  St::phase_diffs(s, a, gi0, gj0, tid, NT);
-  __syncthreads();
-  St::phase_faces(s, a, gi0, gj0, tid, NT);
-  __syncthreads();
-  St::phase_alpha(s, a, gi0, gj0, tid, NT, dt);
-  __syncthreads();
-  double wsx = 0.0, wsy = 0.0;
-  St::template phase_update<MODE>(s, a, gi0, gj0, tid, NT, dt, &wsx, &wsy);
-  if (MODE == hf::MODE_FINAL_RK2 && a.wave_speed) {
-    Ctrl* c = a.ctrl;
-    block_max_to(wsx, wsy, s.red, &c->ws[c->parity ^ 1][0]);
-  }
-}
-
-// Forward-Euler tail and the standalone source steps: elementwise over
-// U_in -> U_out with the optional forcing and CFL maxima.
-template <class T>
-__global__ __launch_bounds__(256) void tail_kernel(StageArgs<T> a, int slot_mode) {
-  __shared__ double red[2][16];
-  double dt;
-  if (!hf::stage_prologue(a, blockIdx.x == 0 && threadIdx.x == 0, &dt)) return;
-  using St = hf::Stage<T, 32, 16>;
-  double wsx = 0.0, wsy = 0.0;
-  const int64_t n = a.nr * a.nc;
-  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < n;
-       idx += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t gi = idx / a.nc, gj = idx % a.nc;
-    T h = a.ih.ld(idx), hu = a.ihu.ld(idx), hv = a.ihv.ld(idx);
-    St::forcing(a, gi, gj, dt, &h, &hu, &hv);
-    a.oh.st(idx, h);
-    a.ohu.st(idx, hu);
-    a.ohv.st(idx, hv);
-    if (a.wave_speed) {
-      T zc = a.z.ld(idx);
-      T zn4[4];
-      zn4[hf::SIDE_N] = (a.nr >= 2 && gi == 0) ? a.z.ld(idx + a.nc) : zc;
-      zn4[hf::SIDE_S] = (a.nr >= 2 && gi == a.nr - 1) ? a.z.ld(idx - a.nc) : zc;
-      zn4[hf::SIDE_W] = (a.nc >= 2 && gj == 0) ? a.z.ld(idx + 1) : zc;
-      zn4[hf::SIDE_E] = (a.nc >= 2 && gj == a.nc - 1) ? a.z.ld(idx - 1) : zc;
-      T man = a.man.ld(idx);
-      St::cell_wave_speed(a, gi, gj, h, hu, hv, zc, zn4, &man, &wsx, &wsy);
-    }
-  }
-  if (a.wave_speed) {
-    Ctrl* c = a.ctrl;
-    block_max_to(wsx, wsy, red, &c->ws[slot_mode ? c->parity : (c->parity ^ 1)][0]);
-  }
-}
-
-// apply_point_sources, sequential in list order (source.rs:52-69).
-template <class T> __global__ void point_source_kernel(StageArgs<T> a, double dt) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  const T Z = hf::cst<T>(0.0);
-  const double cell_area = a.dx * a.dy;
-  for (int k = 0; k < a.n_src; ++k) {
-    const int64_t idx = a.srcs[k].row * a.nc + a.srcs[k].col;
-    const double dh = a.srcs[k].q * dt / cell_area;
-    T h = hf::rmax(a.ih.ld(idx) + dh, Z);
-    T hu = a.ihu.ld(idx), hv = a.ihv.ld(idx);
-    if (hf::val(h) == 0.0) {
-      hu = Z;
-      hv = Z;
-    }
-    a.oh.st(idx, h);
-    a.ohu.st(idx, hu);
-    a.ohv.st(idx, hv);
-  }
-}
-
-__global__ void finalize_kernel(Ctrl* c) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  if (c->status != 0 || c->done != 0) return;
-  c->t = c->t + c->dt;  // sim.rs:225 (plain sum)
-  c->steps = c->steps + 1;
-  c->budget = c->budget - 1;
-  c->parity = c->parity ^ 1;
-}
-
-__global__ void zero_slot_kernel(Ctrl* c) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    c->ws[c->parity][0] = 0ULL;
-    c->ws[c->parity][1] = 0ULL;
-  }
-}
-
-// Plain streaming copy (dst = src), the calibration kernel for HBM counters
-// and the achievable-bandwidth reference.
-__global__ __launch_bounds__(256) void copy_kernel(const double2* __restrict__ src, double2* __restrict__ dst,
-                                                   int64_t n2) {
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n2; i += (int64_t)gridDim.x * blockDim.x)
-    dst[i] = src[i];
-}
-
-__global__ void math_probe_kernel(const double* x, const double* y, int64_t n, double* out) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  double xi = x[i];
-  out[4 * i + 0] = __builtin_sqrt(xi);
-  out[4 * i + 1] = xi / y[i];
-  out[4 * i + 2] = hfm::pow_4_3(xi);
-  out[4 * i + 3] = hfm::pow_1_3m(xi);
-}
-
-}  // namespace
 
 // ============================================================== handle
 struct hf2d_sim {
@@ -211,9 +25,13 @@ struct hf2d_sim {
   double dx = 1.0, dy = 1.0;
   int kind = HF2D_F64;
   hipStream_t stream = nullptr;
-  double* st[2][3] = {{nullptr}};   // state buffers
-  double* tg[2][3] = {{nullptr}};   // tangents (Dual)
-  double *z = nullptr, *zt = nullptr, *man = nullptr, *mant = nullptr;
+  void* st[2][3] = {{nullptr}};   // state buffers (double, or float for HF2D_F32)
+  void* tg[2][3] = {{nullptr}};   // tangents (Dual)
+  void *man = nullptr, *mant = nullptr;
+  double *z = nullptr, *zt = nullptr;  // bed (double / Dual kinds)
+  double* datum = nullptr;             // FP32 kind: FP64 datum per 64x64 block
+  float* zdev = nullptr;               // FP32 kind: FP32 deviation from the datum
+  int64_t datum_cols = 0;
   bool has_zt = false;
   int cur = 0;  // index of the buffer holding the current state
   Ctrl* ctrl = nullptr;
@@ -260,12 +78,7 @@ hf2d_config default_config() {
   return c;
 }
 
-template <class T> Fld<T> fld(const double* v, const double* d);
-template <> Fld<double> fld<double>(const double* v, const double*) { return Fld<double>{v}; }
-template <> Fld<dual> fld<dual>(const double* v, const double* d) { return Fld<dual>{v, d}; }
-template <class T> FldW<T> fldw(double* v, double* d);
-template <> FldW<double> fldw<double>(double* v, double*) { return FldW<double>{v}; }
-template <> FldW<dual> fldw<dual>(double* v, double* d) { return FldW<dual>{v, d}; }
+
 
 // Arguments for a launch reading state buffer `in`, writing buffer `out`,
 // with U^n in buffer `un`.
@@ -286,6 +99,10 @@ template <class T> StageArgs<T> make_args(hf2d_sim* s, int in, int out, int un) 
   a.nhv = fld<T>(s->st[un][2], s->tg[un][2]);
   a.z = fld<T>(s->z, s->has_zt ? s->zt : nullptr);
   a.man = fld<T>(s->man, s->mant);
+  a.datum = s->datum;
+  a.zdev = s->zdev;
+  a.datum_log2 = DATUM_LOG2;
+  a.datum_cols = s->datum_cols;
   for (int k = 0; k < 4; ++k) {
     a.bc_kind[k] = s->bc_kind[k];
     a.bc_param[k] = s->bc_param[k];
@@ -574,6 +391,13 @@ template <class T> int forcing_only(hf2d_sim* s, double dt, int friction, double
   return HF2D_OK;
 }
 
+template <class T> int launch_point_sources(hf2d_sim* s, double dt) {
+  StageArgs<T> a = make_args<T>(s, s->cur, s->cur, s->cur);
+  a.n_src = s->n_src;
+  hipLaunchKernelGGL((point_source_kernel<T>), dim3(1), dim3(64), 0, s->stream, a, dt);
+  return HF2D_OK;
+}
+
 int upload_sources(hf2d_sim* s, const hf2d_point_source* src, int64_t k) {
   for (int64_t i = 0; i < k; ++i)
     if (src[i].row < 0 || src[i].row >= s->nr || src[i].col < 0 || src[i].col >= s->nc)
@@ -596,13 +420,18 @@ int upload_sources(hf2d_sim* s, const hf2d_point_source* src, int64_t k) {
 }
 }  // namespace
 
+#define HF_DISPATCH(s, FN, ...)                                  \
+  ((s)->kind == HF2D_DUAL  ? FN<dual>(__VA_ARGS__)                \
+   : (s)->kind == HF2D_F32 ? FN<float>(__VA_ARGS__)               \
+                           : FN<double>(__VA_ARGS__))
+
 // ============================================================== C ABI
 extern "C" {
 
 int hf2d_create(int64_t n_rows, int64_t n_cols, double dx, double dy, int32_t scalar_kind, hf2d_sim** out) {
   if (!out) return HF2D_INVALID_ARGUMENT;
   *out = nullptr;
-  if (scalar_kind != HF2D_F64 && scalar_kind != HF2D_DUAL) return HF2D_INVALID_ARGUMENT;
+  if (scalar_kind != HF2D_F64 && scalar_kind != HF2D_DUAL && scalar_kind != HF2D_F32) return HF2D_INVALID_ARGUMENT;
   if (!(dx > 0.0) || !(dy > 0.0) || n_rows <= 0 || n_cols <= 0) return HF2D_INVALID_CONFIG;
   hf2d_sim* s = new hf2d_sim();
   s->nr = n_rows;
@@ -611,34 +440,39 @@ int hf2d_create(int64_t n_rows, int64_t n_cols, double dx, double dy, int32_t sc
   s->dy = dy;
   s->kind = scalar_kind;
   s->cfg = default_config();
-  const size_t bytes = (size_t)(n_rows * n_cols) * sizeof(double);
+  const size_t n = (size_t)(n_rows * n_cols);
+  const size_t bytes = n * elem_bytes(scalar_kind);
   auto cleanup = [&](int code) {
     hf2d_destroy(s);
     return code;
   };
+  auto zalloc = [&](void** p, size_t nb) {
+    return hipMalloc(p, nb) == hipSuccess && hipMemset(*p, 0, nb) == hipSuccess;
+  };
   if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) return cleanup(HF2D_HIP_ERROR);
   for (int b = 0; b < 2; ++b)
     for (int f = 0; f < 3; ++f) {
-      if (hipMalloc(&s->st[b][f], bytes) != hipSuccess) return cleanup(HF2D_HIP_ERROR);
-      if (hipMemset(s->st[b][f], 0, bytes) != hipSuccess) return cleanup(HF2D_HIP_ERROR);
-      if (scalar_kind == HF2D_DUAL) {
-        if (hipMalloc(&s->tg[b][f], bytes) != hipSuccess) return cleanup(HF2D_HIP_ERROR);
-        if (hipMemset(s->tg[b][f], 0, bytes) != hipSuccess) return cleanup(HF2D_HIP_ERROR);
-      }
+      if (!zalloc(&s->st[b][f], bytes)) return cleanup(HF2D_HIP_ERROR);
+      if (scalar_kind == HF2D_DUAL && !zalloc(&s->tg[b][f], bytes)) return cleanup(HF2D_HIP_ERROR);
     }
-  if (hipMalloc(&s->z, bytes) != hipSuccess || hipMemset(s->z, 0, bytes) != hipSuccess) return cleanup(HF2D_HIP_ERROR);
-  if (hipMalloc(&s->man, bytes) != hipSuccess || hipMemset(s->man, 0, bytes) != hipSuccess)
-    return cleanup(HF2D_HIP_ERROR);
+  if (!zalloc(&s->man, bytes)) return cleanup(HF2D_HIP_ERROR);
+  if (scalar_kind == HF2D_F32) {
+    const int64_t rows = (n_rows + (1 << DATUM_LOG2) - 1) >> DATUM_LOG2;
+    s->datum_cols = (n_cols + (1 << DATUM_LOG2) - 1) >> DATUM_LOG2;
+    if (!zalloc((void**)&s->datum, (size_t)(rows * s->datum_cols) * sizeof(double))) return cleanup(HF2D_HIP_ERROR);
+    if (!zalloc((void**)&s->zdev, n * sizeof(float))) return cleanup(HF2D_HIP_ERROR);
+  } else {
+    if (!zalloc((void**)&s->z, n * sizeof(double))) return cleanup(HF2D_HIP_ERROR);
+  }
   if (scalar_kind == HF2D_DUAL) {
-    if (hipMalloc(&s->zt, bytes) != hipSuccess || hipMemset(s->zt, 0, bytes) != hipSuccess)
-      return cleanup(HF2D_HIP_ERROR);
-    if (hipMalloc(&s->mant, bytes) != hipSuccess || hipMemset(s->mant, 0, bytes) != hipSuccess)
-      return cleanup(HF2D_HIP_ERROR);
+    if (!zalloc((void**)&s->zt, bytes)) return cleanup(HF2D_HIP_ERROR);
+    if (!zalloc(&s->mant, bytes)) return cleanup(HF2D_HIP_ERROR);
   }
   if (hipMalloc(&s->ctrl, sizeof(Ctrl)) != hipSuccess) return cleanup(HF2D_HIP_ERROR);
   Ctrl c{};
   c.t_target = 0.0;
   c.dt_cap = __builtin_inf();
+  c.compensated = scalar_kind == HF2D_F32 ? 1 : 0;
   if (hipMemcpy(s->ctrl, &c, sizeof(Ctrl), hipMemcpyHostToDevice) != hipSuccess) return cleanup(HF2D_HIP_ERROR);
   *out = s;
   return HF2D_OK;
@@ -652,12 +486,9 @@ int hf2d_destroy(hf2d_sim* s) {
       if (s->st[b][f]) (void)hipFree(s->st[b][f]);
       if (s->tg[b][f]) (void)hipFree(s->tg[b][f]);
     }
-  if (s->z) (void)hipFree(s->z);
-  if (s->zt) (void)hipFree(s->zt);
-  if (s->man) (void)hipFree(s->man);
-  if (s->mant) (void)hipFree(s->mant);
-  if (s->ctrl) (void)hipFree(s->ctrl);
-  if (s->srcs) (void)hipFree(s->srcs);
+  void* bufs[] = {s->z, s->zt, s->man, s->mant, s->datum, s->zdev, s->ctrl, s->srcs};
+  for (void* b : bufs)
+    if (b) (void)hipFree(b);
   for (hipEvent_t e : s->ev) (void)hipEventDestroy(e);
   if (s->stream) (void)hipStreamDestroy(s->stream);
   delete s;
@@ -672,9 +503,34 @@ int hf2d_upload_mesh(hf2d_sim* s, const double* bed, const double* bed_tangent, 
   const size_t n = (size_t)(s->nr * s->nc);
   for (size_t k = 0; k < n; ++k)
     if (!(manning[k] >= 0.0)) return fail(s, HF2D_INVALID_CONFIG, "Manning n must be non-negative at every cell");
+  HIPCHK(s, hipStreamSynchronize(s->stream));
   const size_t bytes = n * sizeof(double);
-  HIPCHK(s, hipMemcpy(s->z, bed, bytes, hipMemcpyHostToDevice));
-  HIPCHK(s, hipMemcpy(s->man, manning, bytes, hipMemcpyHostToDevice));
+  if (s->kind == HF2D_F32) {
+    // Datum per 64x64 block: the block mean in FP64; deviation rounded to FP32.
+    const int64_t B = 1 << DATUM_LOG2, rows = (s->nr + B - 1) / B;
+    std::vector<double> dat((size_t)(rows * s->datum_cols), 0.0);
+    for (int64_t bi = 0; bi < rows; ++bi)
+      for (int64_t bj = 0; bj < s->datum_cols; ++bj) {
+        double sum = 0.0;
+        int64_t cnt = 0;
+        for (int64_t i = bi * B; i < std::min(s->nr, (bi + 1) * B); ++i)
+          for (int64_t j = bj * B; j < std::min(s->nc, (bj + 1) * B); ++j, ++cnt) sum += bed[i * s->nc + j];
+        dat[(size_t)(bi * s->datum_cols + bj)] = sum / (double)cnt;
+      }
+    std::vector<float> dev(n), man(n);
+    for (int64_t i = 0; i < s->nr; ++i)
+      for (int64_t j = 0; j < s->nc; ++j) {
+        const size_t k = (size_t)(i * s->nc + j);
+        dev[k] = (float)(bed[k] - dat[(size_t)((i >> DATUM_LOG2) * s->datum_cols + (j >> DATUM_LOG2))]);
+        man[k] = (float)manning[k];
+      }
+    HIPCHK(s, hipMemcpy(s->datum, dat.data(), dat.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(s, hipMemcpy(s->zdev, dev.data(), n * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(s, hipMemcpy(s->man, man.data(), n * sizeof(float), hipMemcpyHostToDevice));
+  } else {
+    HIPCHK(s, hipMemcpy(s->z, bed, bytes, hipMemcpyHostToDevice));
+    HIPCHK(s, hipMemcpy(s->man, manning, bytes, hipMemcpyHostToDevice));
+  }
   if (s->kind == HF2D_DUAL) {
     s->has_zt = bed_tangent != nullptr;
     if (bed_tangent)
@@ -690,17 +546,45 @@ int hf2d_upload_mesh(hf2d_sim* s, const double* bed, const double* bed_tangent, 
   return HF2D_OK;
 }
 
+namespace {
+// Copy one AoS component (stride 3) into a device plane of the grid's kind.
+int put_plane(hf2d_sim* s, void* dst, const double* aos, int f) {
+  const size_t n = (size_t)(s->nr * s->nc);
+  if (s->kind == HF2D_F32) {
+    std::vector<float> plane(n);
+    for (size_t k = 0; k < n; ++k) plane[k] = aos ? (float)aos[3 * k + f] : 0.0f;
+    HIPCHK(s, hipMemcpy(dst, plane.data(), n * sizeof(float), hipMemcpyHostToDevice));
+  } else {
+    std::vector<double> plane(n);
+    for (size_t k = 0; k < n; ++k) plane[k] = aos ? aos[3 * k + f] : 0.0;
+    HIPCHK(s, hipMemcpy(dst, plane.data(), n * sizeof(double), hipMemcpyHostToDevice));
+  }
+  return HF2D_OK;
+}
+int get_plane(hf2d_sim* s, const void* src, double* aos, int f) {
+  const size_t n = (size_t)(s->nr * s->nc);
+  if (s->kind == HF2D_F32) {
+    std::vector<float> plane(n);
+    HIPCHK(s, hipMemcpy(plane.data(), src, n * sizeof(float), hipMemcpyDeviceToHost));
+    for (size_t k = 0; k < n; ++k) aos[3 * k + f] = (double)plane[k];
+  } else {
+    std::vector<double> plane(n);
+    HIPCHK(s, hipMemcpy(plane.data(), src, n * sizeof(double), hipMemcpyDeviceToHost));
+    for (size_t k = 0; k < n; ++k) aos[3 * k + f] = plane[k];
+  }
+  return HF2D_OK;
+}
+}  // namespace
+
 int hf2d_upload_state(hf2d_sim* s, const double* aos, const double* tan_aos) {
   if (!valid(s) || !aos) return fail(s, HF2D_INVALID_ARGUMENT, "null state array");
   HIPCHK(s, hipStreamSynchronize(s->stream));
-  const size_t n = (size_t)(s->nr * s->nc);
-  std::vector<double> plane(n);
   for (int f = 0; f < 3; ++f) {
-    for (size_t k = 0; k < n; ++k) plane[k] = aos[3 * k + f];
-    HIPCHK(s, hipMemcpy(s->st[s->cur][f], plane.data(), n * sizeof(double), hipMemcpyHostToDevice));
+    int rc = put_plane(s, s->st[s->cur][f], aos, f);
+    if (rc) return rc;
     if (s->kind == HF2D_DUAL) {
-      for (size_t k = 0; k < n; ++k) plane[k] = tan_aos ? tan_aos[3 * k + f] : 0.0;
-      HIPCHK(s, hipMemcpy(s->tg[s->cur][f], plane.data(), n * sizeof(double), hipMemcpyHostToDevice));
+      rc = put_plane(s, s->tg[s->cur][f], tan_aos, f);
+      if (rc) return rc;
     }
   }
   s->ws_valid = false;
@@ -710,16 +594,45 @@ int hf2d_upload_state(hf2d_sim* s, const double* aos, const double* tan_aos) {
 int hf2d_download_state(hf2d_sim* s, double* aos, double* tan_aos) {
   if (!valid(s) || !aos) return fail(s, HF2D_INVALID_ARGUMENT, "null state array");
   HIPCHK(s, hipStreamSynchronize(s->stream));
-  const size_t n = (size_t)(s->nr * s->nc);
-  std::vector<double> plane(n);
   for (int f = 0; f < 3; ++f) {
-    HIPCHK(s, hipMemcpy(plane.data(), s->st[s->cur][f], n * sizeof(double), hipMemcpyDeviceToHost));
-    for (size_t k = 0; k < n; ++k) aos[3 * k + f] = plane[k];
+    int rc = get_plane(s, s->st[s->cur][f], aos, f);
+    if (rc) return rc;
     if (s->kind == HF2D_DUAL && tan_aos) {
-      HIPCHK(s, hipMemcpy(plane.data(), s->tg[s->cur][f], n * sizeof(double), hipMemcpyDeviceToHost));
-      for (size_t k = 0; k < n; ++k) tan_aos[3 * k + f] = plane[k];
+      rc = get_plane(s, s->tg[s->cur][f], tan_aos, f);
+      if (rc) return rc;
     }
   }
+  return HF2D_OK;
+}
+
+int hf2d_total_volume(hf2d_sim* s, double* volume) {
+  if (!valid(s) || !volume) return fail(s, HF2D_INVALID_ARGUMENT, "null argument");
+  const int64_t n = s->nr * s->nc;
+  const int nb = 1024;
+  double* part = nullptr;
+  HIPCHK(s, hipMalloc(&part, 2 * nb * sizeof(double)));
+  if (s->kind == HF2D_DUAL)
+    hipLaunchKernelGGL(volume_kernel<dual>, dim3(nb), dim3(256), 0, s->stream,
+                       fld<dual>(s->st[s->cur][0], s->tg[s->cur][0]), n, part);
+  else if (s->kind == HF2D_F32)
+    hipLaunchKernelGGL(volume_kernel<float>, dim3(nb), dim3(256), 0, s->stream, fld<float>(s->st[s->cur][0], nullptr),
+                       n, part);
+  else
+    hipLaunchKernelGGL(volume_kernel<double>, dim3(nb), dim3(256), 0, s->stream,
+                       fld<double>(s->st[s->cur][0], nullptr), n, part);
+  std::vector<double> hp(2 * nb);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(hp.data(), part, 2 * nb * sizeof(double), hipMemcpyDeviceToHost, s->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+  (void)hipFree(part);
+  if (e != hipSuccess) return fail(s, HF2D_HIP_ERROR, hipGetErrorString(e));
+  double sum = 0.0, comp = 0.0;  // Neumaier over the block pairs, fixed order
+  for (int k = 0; k < 2 * nb; ++k) {
+    const double x = hp[k], t = sum + x;
+    comp += (__builtin_fabs(sum) >= __builtin_fabs(x)) ? (sum - t) + x : (x - t) + sum;
+    sum = t;
+  }
+  *volume = (sum + comp) * s->dx * s->dy;
   return HF2D_OK;
 }
 
@@ -737,15 +650,13 @@ int hf2d_set_boundaries(hf2d_sim* s, const int32_t kinds[4], const double params
 
 int hf2d_max_wave_speeds(hf2d_sim* s, double* sx, double* sy) {
   if (!valid(s) || !sx || !sy) return fail(s, HF2D_INVALID_ARGUMENT, "null argument");
-  return s->kind == HF2D_DUAL ? do_max_wave_speeds<dual>(s, false, sx, sy)
-                              : do_max_wave_speeds<double>(s, false, sx, sy);
+  return HF_DISPATCH(s, do_max_wave_speeds, s, false, sx, sy);
 }
 
 int hf2d_cfl_time_step(hf2d_sim* s, double cfl, int32_t with_bcs, double* dt) {
   if (!valid(s) || !dt) return fail(s, HF2D_INVALID_ARGUMENT, "null argument");
   double sx, sy;
-  int rc = s->kind == HF2D_DUAL ? do_max_wave_speeds<dual>(s, with_bcs != 0, &sx, &sy)
-                                : do_max_wave_speeds<double>(s, with_bcs != 0, &sx, &sy);
+  int rc = HF_DISPATCH(s, do_max_wave_speeds, s, with_bcs != 0, &sx, &sy);
   if (rc) return rc;
   const double denom = sx / s->dx + sy / s->dy;  // update.rs:150-155
   *dt = denom > 0.0 ? cfl / denom : __builtin_inf();
@@ -754,26 +665,24 @@ int hf2d_cfl_time_step(hf2d_sim* s, double cfl, int32_t with_bcs, double* dt) {
 
 int hf2d_forward_euler_step(hf2d_sim* s, double dt) {
   if (!valid(s)) return HF2D_INVALID_ARGUMENT;
-  return s->kind == HF2D_DUAL ? host_dt_step<dual>(s, dt, false) : host_dt_step<double>(s, dt, false);
+  return HF_DISPATCH(s, host_dt_step, s, dt, false);
 }
 
 int hf2d_ssprk2_step(hf2d_sim* s, double dt) {
   if (!valid(s)) return HF2D_INVALID_ARGUMENT;
-  return s->kind == HF2D_DUAL ? host_dt_step<dual>(s, dt, true) : host_dt_step<double>(s, dt, true);
+  return HF_DISPATCH(s, host_dt_step, s, dt, true);
 }
 
 
 int hf2d_manning_friction_step(hf2d_sim* s, double dt, double dry_tol) {
   if (!valid(s)) return HF2D_INVALID_ARGUMENT;
-  return s->kind == HF2D_DUAL ? forcing_only<dual>(s, dt, 1, dry_tol, 0.0)
-                              : forcing_only<double>(s, dt, 1, dry_tol, 0.0);
+  return HF_DISPATCH(s, forcing_only, s, dt, 1, dry_tol, 0.0);
 }
 
 int hf2d_apply_rain(hf2d_sim* s, double rate, double dt) {
   if (!valid(s)) return HF2D_INVALID_ARGUMENT;
   if (rate == 0.0) return HF2D_OK;  // source.rs:89-91
-  return s->kind == HF2D_DUAL ? forcing_only<dual>(s, dt, 0, 0.0, rate)
-                              : forcing_only<double>(s, dt, 0, 0.0, rate);
+  return HF_DISPATCH(s, forcing_only, s, dt, 0, 0.0, rate);
 }
 
 int hf2d_apply_point_sources(hf2d_sim* s, const hf2d_point_source* src, int64_t k, double dt) {
@@ -789,15 +698,7 @@ int hf2d_apply_point_sources(hf2d_sim* s, const hf2d_point_source* src, int64_t 
   s->cap_src = 0;
   rc = upload_sources(s, src, k);
   if (rc == HF2D_OK) {
-    if (s->kind == HF2D_DUAL) {
-      StageArgs<dual> a = make_args<dual>(s, s->cur, s->cur, s->cur);
-      a.n_src = s->n_src;
-      hipLaunchKernelGGL((point_source_kernel<dual>), dim3(1), dim3(64), 0, s->stream, a, dt);
-    } else {
-      StageArgs<double> a = make_args<double>(s, s->cur, s->cur, s->cur);
-      a.n_src = s->n_src;
-      hipLaunchKernelGGL((point_source_kernel<double>), dim3(1), dim3(64), 0, s->stream, a, dt);
-    }
+    HF_DISPATCH(s, launch_point_sources, s, dt);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) rc = fail(s, HF2D_HIP_ERROR, hipGetErrorString(e));
     hipStreamSynchronize(s->stream);
@@ -834,8 +735,7 @@ int hf2d_configure(hf2d_sim* s, const hf2d_config* cfg, const hf2d_point_source*
 
 int hf2d_step(hf2d_sim* s, double dt_cap, double* dt_taken) {
   if (!valid(s)) return HF2D_INVALID_ARGUMENT;
-  int rc = s->kind == HF2D_DUAL ? run_driver<dual>(s, false, 0.0, dt_cap, 1, true)
-                                : run_driver<double>(s, false, 0.0, dt_cap, 1, true);
+  int rc = HF_DISPATCH(s, run_driver, s, false, 0.0, dt_cap, 1, true);
   if (dt_taken) {
     Ctrl c;
     if (read_ctrl(s, &c) == HF2D_OK) *dt_taken = c.dt;
@@ -845,22 +745,19 @@ int hf2d_step(hf2d_sim* s, double dt_cap, double* dt_taken) {
 
 int hf2d_run_until(hf2d_sim* s, double t_target) {
   if (!valid(s)) return HF2D_INVALID_ARGUMENT;
-  return s->kind == HF2D_DUAL ? run_driver<dual>(s, true, t_target, 0.0, 0, true)
-                              : run_driver<double>(s, true, t_target, 0.0, 0, true);
+  return HF_DISPATCH(s, run_driver, s, true, t_target, 0.0, 0, true);
 }
 
 int hf2d_run_steps(hf2d_sim* s, int64_t n) {
   if (!valid(s)) return HF2D_INVALID_ARGUMENT;
   if (n <= 0) return HF2D_OK;
-  return s->kind == HF2D_DUAL ? run_driver<dual>(s, false, 0.0, __builtin_inf(), n, true)
-                              : run_driver<double>(s, false, 0.0, __builtin_inf(), n, true);
+  return HF_DISPATCH(s, run_driver, s, false, 0.0, __builtin_inf(), n, true);
 }
 
 int hf2d_enqueue_steps(hf2d_sim* s, int64_t n) {
   if (!valid(s)) return HF2D_INVALID_ARGUMENT;
   if (n <= 0) return HF2D_OK;
-  return s->kind == HF2D_DUAL ? run_driver<dual>(s, false, 0.0, __builtin_inf(), n, false)
-                              : run_driver<double>(s, false, 0.0, __builtin_inf(), n, false);
+  return HF_DISPATCH(s, run_driver, s, false, 0.0, __builtin_inf(), n, false);
 }
 
 int hf2d_synchronize(hf2d_sim* s) {

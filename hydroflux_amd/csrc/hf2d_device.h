@@ -1,0 +1,328 @@
+// Device kernels shared by the single-grid driver (hf2d.hip) and the batched
+// engine (hf2d_batch.hip): the fused stage kernel, the elementwise tail
+// (forcing + CFL maxima), point sources, the clock finalize, the volume
+// reduction and the measurement probes.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "hf2d_tile.h"
+
+using hf::Ctrl;
+using hf::dual;
+using hf::Fld;
+using hf::FldW;
+using hf::PointSrc;
+using hf::StageArgs;
+
+namespace {
+
+// ------------------------------------------------------------ tile shapes
+template <class T> struct Shape;
+#ifndef HF_F64_TILE_W
+#define HF_F64_TILE_W 32
+#endif
+#ifndef HF_F64_TILE_H
+#define HF_F64_TILE_H 16
+#endif
+#ifndef HF_F64_THREADS
+#define HF_F64_THREADS 1024
+#endif
+#ifndef HF_XCD_REMAP
+#define HF_XCD_REMAP 1
+#endif
+
+// Workgroups are dispatched round-robin over the 8 XCDs (workgroup b lands on
+// XCD b % 8), each with its own L2. Renumber tiles so every XCD walks one
+// contiguous band of tiles: horizontally adjacent tiles, which share halo
+// columns, then run on the same XCD at about the same time and the halo is
+// an L2 hit instead of a second HBM fetch.
+__device__ inline unsigned tile_of_block(unsigned b, unsigned nb) {
+#if HF_XCD_REMAP
+  const unsigned q = nb / 8u, r = nb % 8u, x = b % 8u;
+  return x * q + (x < r ? x : r) + b / 8u;
+#else
+  (void)nb;
+  return b;
+#endif
+}
+template <> struct Shape<double> {
+  static constexpr int TW = HF_F64_TILE_W, TH = HF_F64_TILE_H, NT = HF_F64_THREADS;
+};
+template <> struct Shape<dual> {
+  static constexpr int TW = 16, TH = 16, NT = 256;
+};
+#ifndef HF_F32_TILE_W
+#define HF_F32_TILE_W 32
+#endif
+#ifndef HF_F32_TILE_H
+#define HF_F32_TILE_H 32
+#endif
+#ifndef HF_F32_THREADS
+#define HF_F32_THREADS 1024
+#endif
+template <> struct Shape<float> {
+  static constexpr int TW = HF_F32_TILE_W, TH = HF_F32_TILE_H, NT = HF_F32_THREADS;
+};
+
+__device__ inline void block_max_to(double sx, double sy, double (*red)[16], unsigned long long* dst) {
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    sx = __builtin_fmax(sx, __shfl_xor(sx, off));
+    sy = __builtin_fmax(sy, __shfl_xor(sy, off));
+  }
+  const int wave = tid >> 6, lane = tid & 63;
+  if (lane == 0) {
+    red[0][wave] = sx;
+    red[1][wave] = sy;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    const int nw = (blockDim.x + 63) >> 6;
+    for (int w = 1; w < nw; ++w) {
+      sx = __builtin_fmax(sx, red[0][w]);
+      sy = __builtin_fmax(sy, red[1][w]);
+    }
+    atomicMax(dst + 0, hfm::as_u64(sx));
+    atomicMax(dst + 1, hfm::as_u64(sy));
+  }
+}
+
+template <class T, int MODE>
+__global__ __launch_bounds__(Shape<T>::NT) void stage_kernel(StageArgs<T> a) {
+  constexpr int TW = Shape<T>::TW, TH = Shape<T>::TH, NT = Shape<T>::NT;
+  using St = hf::Stage<T, TW, TH>;
+  __shared__ typename St::S s;
+  double dt;
+  if (!hf::stage_prologue(a, blockIdx.x == 0 && threadIdx.x == 0, &dt)) return;
+  const unsigned tile = tile_of_block(blockIdx.x, gridDim.x);
+  const int64_t ty = tile / a.tiles_x, tx = tile % a.tiles_x;
+  const int64_t gi0 = ty * TH, gj0 = tx * TW;
+  const int tid = threadIdx.x;
+  St::phase_load(s, a, gi0, gj0, tid, NT);
+  __syncthreads();
+  St::phase_diffs(s, a, gi0, gj0, tid, NT);
+  __syncthreads();
+  St::phase_faces(s, a, gi0, gj0, tid, NT);
+  __syncthreads();
+  St::phase_alpha(s, a, gi0, gj0, tid, NT, dt);
+  __syncthreads();
+  double wsx = 0.0, wsy = 0.0;
+  St::template phase_update<MODE>(s, a, gi0, gj0, tid, NT, dt, &wsx, &wsy);
+  if (MODE == hf::MODE_FINAL_RK2 && a.wave_speed) {
+    Ctrl* c = a.ctrl;
+    block_max_to(wsx, wsy, s.red, &c->ws[c->parity ^ 1][0]);
+  }
+}
+
+// Forward-Euler tail and the standalone source steps: elementwise over
+// U_in -> U_out with the optional forcing and CFL maxima.
+template <class T>
+__global__ __launch_bounds__(256) void tail_kernel(StageArgs<T> a, int slot_mode) {
+  __shared__ double red[2][16];
+  double dt;
+  if (!hf::stage_prologue(a, blockIdx.x == 0 && threadIdx.x == 0, &dt)) return;
+  using St = hf::Stage<T, 32, 16>;
+  double wsx = 0.0, wsy = 0.0;
+  const int64_t n = a.nr * a.nc;
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < n;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t gi = idx / a.nc, gj = idx % a.nc;
+    T h = a.ih.ld(idx), hu = a.ihu.ld(idx), hv = a.ihv.ld(idx);
+    St::forcing(a, gi, gj, dt, &h, &hu, &hv);
+    a.oh.st(idx, h);
+    a.ohu.st(idx, hu);
+    a.ohv.st(idx, hv);
+    if (a.wave_speed) {
+      const double td = hf::tile_datum(a, gi, gj);
+      T zc = hf::load_bed(a, gi, gj, idx, td);
+      T zn4[4];
+      zn4[hf::SIDE_N] = (a.nr >= 2 && gi == 0) ? hf::load_bed(a, gi + 1, gj, idx + a.nc, td) : zc;
+      zn4[hf::SIDE_S] = (a.nr >= 2 && gi == a.nr - 1) ? hf::load_bed(a, gi - 1, gj, idx - a.nc, td) : zc;
+      zn4[hf::SIDE_W] = (a.nc >= 2 && gj == 0) ? hf::load_bed(a, gi, gj + 1, idx + 1, td) : zc;
+      zn4[hf::SIDE_E] = (a.nc >= 2 && gj == a.nc - 1) ? hf::load_bed(a, gi, gj - 1, idx - 1, td) : zc;
+      T man = a.man.ld(idx);
+      St::cell_wave_speed(a, gi, gj, h, hu, hv, zc, zn4, &man, &wsx, &wsy);
+    }
+  }
+  if (a.wave_speed) {
+    Ctrl* c = a.ctrl;
+    block_max_to(wsx, wsy, red, &c->ws[slot_mode ? c->parity : (c->parity ^ 1)][0]);
+  }
+}
+
+// apply_point_sources, sequential in list order (source.rs:52-69).
+template <class T> __global__ void point_source_kernel(StageArgs<T> a, double dt) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  using R = typename hf::ScalarOf<T>::type;
+  const T Z = hf::cst<T>(0.0);
+  const double cell_area = a.dx * a.dy;
+  for (int k = 0; k < a.n_src; ++k) {
+    const int64_t idx = a.srcs[k].row * a.nc + a.srcs[k].col;
+    const R dh = R(a.srcs[k].q * dt / cell_area);
+    T h = hf::rmax(a.ih.ld(idx) + dh, Z);
+    T hu = a.ihu.ld(idx), hv = a.ihv.ld(idx);
+    if (hf::val(h) == R(0)) {
+      hu = Z;
+      hv = Z;
+    }
+    a.oh.st(idx, h);
+    a.ohu.st(idx, hu);
+    a.ohv.st(idx, hv);
+  }
+}
+
+__global__ void finalize_kernel(Ctrl* c) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  if (c->status != 0 || c->done != 0) return;
+  if (c->compensated) {  // FP32 precision form: Kahan-compensated clock
+    const double y = c->dt - c->t_comp;
+    const double t1 = c->t + y;
+    c->t_comp = (t1 - c->t) - y;
+    c->t = t1;
+  } else {
+    c->t = c->t + c->dt;  // sim.rs:225 (plain sum)
+  }
+  c->steps = c->steps + 1;
+  c->budget = c->budget - 1;
+  c->parity = c->parity ^ 1;
+}
+
+__global__ void zero_slot_kernel(Ctrl* c) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    c->ws[c->parity][0] = 0ULL;
+    c->ws[c->parity][1] = 0ULL;
+  }
+}
+
+// Plain streaming copy (dst = src), the calibration kernel for HBM counters
+// and the achievable-bandwidth reference.
+__global__ __launch_bounds__(256) void copy_kernel(const double2* __restrict__ src, double2* __restrict__ dst,
+                                                   int64_t n2) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n2; i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
+__global__ void math_probe_kernel(const double* x, const double* y, int64_t n, double* out) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double xi = x[i];
+  out[4 * i + 0] = __builtin_sqrt(xi);
+  out[4 * i + 1] = xi / y[i];
+  out[4 * i + 2] = hfm::pow_4_3(xi);
+  out[4 * i + 3] = hfm::pow_1_3m(xi);
+}
+
+// Total water volume sum(h) * dx * dy in FP64: per-thread Kahan sums over a
+// grid-stride range, then a fixed-order pairwise combination in LDS; one
+// (sum, compensation) pair per block, combined on the host. Deterministic
+// for a given grid size.
+template <class T>
+__global__ __launch_bounds__(256) void volume_kernel(Fld<T> h, int64_t n, double* partial) {
+  __shared__ double ss[256], cc[256];
+  double sum = 0.0, comp = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const double y = (double)hf::val(h.ld(i)) - comp;
+    const double t = sum + y;
+    comp = (t - sum) - y;
+    sum = t;
+  }
+  ss[threadIdx.x] = sum;
+  cc[threadIdx.x] = -comp;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      const double a = ss[threadIdx.x], b = ss[threadIdx.x + w];
+      const double t = a + b;  // TwoSum
+      const double bp = t - a;
+      const double err = (a - (t - bp)) + (b - bp);
+      ss[threadIdx.x] = t;
+      cc[threadIdx.x] = cc[threadIdx.x] + cc[threadIdx.x + w] + err;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    partial[2 * blockIdx.x] = ss[0];
+    partial[2 * blockIdx.x + 1] = cc[0];
+  }
+}
+
+
+// ------------------------------------------------------------ host-side field views
+template <class T> Fld<T> fld(const void* v, const void* d);
+template <> Fld<double> fld<double>(const void* v, const void*) { return Fld<double>{(const double*)v}; }
+template <> Fld<dual> fld<dual>(const void* v, const void* d) {
+  return Fld<dual>{(const double*)v, (const double*)d};
+}
+template <> Fld<float> fld<float>(const void* v, const void*) { return Fld<float>{(const float*)v}; }
+template <class T> FldW<T> fldw(void* v, void* d);
+template <> FldW<double> fldw<double>(void* v, void*) { return FldW<double>{(double*)v}; }
+template <> FldW<dual> fldw<dual>(void* v, void* d) { return FldW<dual>{(double*)v, (double*)d}; }
+template <> FldW<float> fldw<float>(void* v, void*) { return FldW<float>{(float*)v}; }
+
+constexpr int DATUM_LOG2 = 6;  // FP32 kind: one FP64 bed datum per 64 x 64 cells
+
+inline size_t elem_bytes(int kind) { return kind == 2 /* HF2D_F32 */ ? sizeof(float) : sizeof(double); }
+
+// ------------------------------------------------------------ batched engine
+// One work item = one output tile of one scenario.
+struct WorkItem {
+  int slot;
+  int tile;
+};
+
+// Persistent stage kernel over a device work list spanning many scenarios
+// of different sizes. Each scenario carries its own StageArgs (pointers,
+// shape, boundaries, forcing) and its own Ctrl (clock, dt, CFL slots), so
+// every scenario advances with its own dt; a scenario whose clock reached
+// its end time (or failed) makes its tiles no-ops.
+template <class T, int TW, int TH, int NT, int MODE>
+__global__ __launch_bounds__(NT) void batch_stage_kernel(const StageArgs<T>* __restrict__ descs,
+                                                         const WorkItem* __restrict__ work, int n_work) {
+  using St = hf::Stage<T, TW, TH>;
+  __shared__ typename St::S s;
+  const int tid = threadIdx.x;
+  for (int w = blockIdx.x; w < n_work; w += gridDim.x) {
+    const WorkItem it = work[w];
+    const StageArgs<T>& a = descs[it.slot];
+    double dt;
+    if (!hf::stage_prologue(a, it.tile == 0 && tid == 0, &dt)) continue;
+    const int64_t gi0 = (int64_t)(it.tile / a.tiles_x) * TH, gj0 = (int64_t)(it.tile % a.tiles_x) * TW;
+    St::phase_load(s, a, gi0, gj0, tid, NT);
+    __syncthreads();
+    St::phase_diffs(s, a, gi0, gj0, tid, NT);
+    __syncthreads();
+    St::phase_faces(s, a, gi0, gj0, tid, NT);
+    __syncthreads();
+    St::phase_alpha(s, a, gi0, gj0, tid, NT, dt);
+    __syncthreads();
+    double wsx = 0.0, wsy = 0.0;
+    St::template phase_update<MODE>(s, a, gi0, gj0, tid, NT, dt, &wsx, &wsy);
+    if (MODE == hf::MODE_FINAL_RK2 && a.wave_speed) {
+      Ctrl* c = a.ctrl;
+      block_max_to(wsx, wsy, s.red, &c->ws[c->parity ^ 1][0]);
+    }
+    __syncthreads();  // the next work item overwrites the LDS tile
+  }
+}
+
+// Clock update of every scenario in the batch (finalize_kernel per slot).
+__global__ void batch_finalize_kernel(Ctrl* ctrls, const int* slots, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  Ctrl* c = ctrls + slots[i];
+  if (c->status != 0 || c->done != 0) return;
+  if (c->compensated) {
+    const double y = c->dt - c->t_comp;
+    const double t1 = c->t + y;
+    c->t_comp = (t1 - c->t) - y;
+    c->t = t1;
+  } else {
+    c->t = c->t + c->dt;
+  }
+  c->steps = c->steps + 1;
+  c->budget = c->budget - 1;
+  c->parity = c->parity ^ 1;
+}
+
+}  // namespace

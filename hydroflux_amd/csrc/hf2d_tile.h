@@ -56,7 +56,15 @@ template <> struct Fld<dual> {
   const double* d;  // may be null: tangent 0 (constant bed)
   HF_HD dual ld(int64_t k) const { return dual{v[k], d ? d[k] : 0.0}; }
 };
+template <> struct Fld<float> {
+  const float* v;
+  HF_HD float ld(int64_t k) const { return v[k]; }
+};
 template <class T> struct FldW;
+template <> struct FldW<float> {
+  float* v;
+  HF_HD void st(int64_t k, float x) const { v[k] = x; }
+};
 template <> struct FldW<double> {
   double* v;
   HF_HD void st(int64_t k, double x) const { v[k] = x; }
@@ -85,6 +93,9 @@ struct Ctrl {
   int done;    // run_until reached its target
   int parity;  // ws[parity] holds the maxima of the current state
   int use_target;
+  double t_comp;    // Kahan compensation of t (FP32 precision form)
+  int compensated;  // 1: compensated clock; 0: plain t += dt (reference)
+  int pad_;
 };
 
 struct PointSrc {  // source.rs:36-44
@@ -114,27 +125,53 @@ template <class T> struct StageArgs {
   const PointSrc* srcs;
   int wave_speed;  // reduce the output's wave speeds into ctrl->ws[parity ^ 1]
   int tiles_x, tiles_y;
+  // FP32 precision form: bed = datum[block] (FP64, one per 2^datum_log2
+  // square block of cells) + zdev (FP32 deviation); `z` is unused then.
+  const double* datum;
+  const float* zdev;
+  int datum_log2;
+  int64_t datum_cols;
 };
+
+// Bed of one cell in the tile's frame. double/dual: the stored bed. float:
+// the FP64 datum of the cell's block minus the tile datum, plus the FP32
+// deviation, summed in FP64 and rounded once, so a tile on 2000 m relief
+// works with beds of a few metres and h + z keeps its FP32 digits.
+template <class T> HF_HD T load_bed(const StageArgs<T>& a, int64_t gi, int64_t gj, int64_t idx, double) {
+  return a.z.ld(idx);
+}
+template <> HF_HD float load_bed<float>(const StageArgs<float>& a, int64_t gi, int64_t gj, int64_t idx,
+                                        double tile_datum) {
+  const double d = a.datum[(gi >> a.datum_log2) * a.datum_cols + (gj >> a.datum_log2)];
+  return (float)((d - tile_datum) + (double)a.zdev[idx]);
+}
+template <class T> HF_HD double tile_datum(const StageArgs<T>&, int64_t, int64_t) { return 0.0; }
+template <> HF_HD double tile_datum<float>(const StageArgs<float>& a, int64_t gi, int64_t gj) {
+  return a.datum[(gi >> a.datum_log2) * a.datum_cols + (gj >> a.datum_log2)];
+}
 
 // ------------------------------------------------------------ small pieces
 // update.rs:362-370
 template <class T> HF_HD T minmod(T a, T b) {
-  if (val(a) * val(b) <= 0.0) return cst<T>(0.0);
-  if (__builtin_fabs(val(a)) < __builtin_fabs(val(b))) return a;
+  using R = typename ScalarOf<T>::type;
+  if (val(a) * val(b) <= R(0)) return cst<T>(0.0);
+  if (rabs(val(a)) < rabs(val(b))) return a;
   return b;
 }
 
 // update.rs:650-656
 template <class T> HF_HD T interior_z_face(bool l_dry, T z_l, bool r_dry, T z_r) {
+  using R = typename ScalarOf<T>::type;
   if (l_dry != r_dry) return rmax(z_l, z_r);
-  return (z_l + z_r) * 0.5;
+  return (z_l + z_r) * R(0.5);
 }
 
 // boundary.rs:335-367. zc: bed of the boundary cell, zn: bed of the next
 // inner cell along the face normal.
 template <class T> HF_HD T ghost_bed(int kind, T zc, T zn) {
+  using R = typename ScalarOf<T>::type;
   if (kind == BC_TRANSMISSIVE || kind == BC_WALL) return zc;
-  return zc * 2.0 - zn;
+  return zc * R(2.0) - zn;
 }
 
 // boundary.rs:193-219 + 274-333. `d` is dx for West/East, dy for
@@ -142,6 +179,7 @@ template <class T> HF_HD T ghost_bed(int kind, T zc, T zn) {
 template <class T>
 HF_HD void ghost_state(int kind, double param, int side, T h, T hu, T hv, T zc, T zn, bool have_next,
                        double d, T man, T* gh, T* ghu, T* ghv) {
+  using R = typename ScalarOf<T>::type;
   const bool xf = (side == SIDE_W || side == SIDE_E);
   *gh = h;
   *ghu = hu;
@@ -156,11 +194,11 @@ HF_HD void ghost_state(int kind, double param, int side, T h, T hu, T hv, T zc, 
       *ghu = cst<T>(param);
     else
       *ghv = cst<T>(param);
-    if (val(h) < H_DRY) {
+    if (val(h) < R(H_DRY)) {
       // bed_slope_x/y (geometry.rs:131-146), magnitude only
-      double slope = have_next ? __builtin_fabs(val((zc - zn) / d)) : 0.0;
+      double slope = have_next ? __builtin_fabs((double)val(zc - zn) / d) : 0.0;
       if (slope > DRY_INFLOW_SLOPE_THRESHOLD) {
-        T hn = powf_gen(man * (__builtin_fabs(param) / __builtin_sqrt(slope)), 3.0 / 5.0);
+        T hn = powf_gen(man * R(__builtin_fabs(param) / __builtin_sqrt(slope)), 3.0 / 5.0);
         *gh = rmin(hn, cst<T>(DRY_INFLOW_H_MAX));
       }
     }
@@ -172,8 +210,9 @@ HF_HD void ghost_state(int kind, double param, int side, T h, T hu, T hv, T zc, 
 // riemann.rs:114-204
 template <class T>
 HF_HD void hllc_normal_flux(T h_l, T qn_l, T qt_l, T h_r, T qn_r, T qt_r, T* fm, T* fn, T* ft) {
-  const bool lw = val(h_l) > DRY_TOL;
-  const bool rw = val(h_r) > DRY_TOL;
+  using R = typename ScalarOf<T>::type;
+  const bool lw = val(h_l) > R(DRY_TOL);
+  const bool rw = val(h_r) > R(DRY_TOL);
   const T Z = cst<T>(0.0);
   if (!lw && !rw) {
     *fm = Z;
@@ -181,27 +220,27 @@ HF_HD void hllc_normal_flux(T h_l, T qn_l, T qt_l, T h_r, T qn_r, T qt_r, T* fm,
     *ft = Z;
     return;
   }
-  T cl = lw ? rsqrt_(h_l * GRAVITY) : Z;
-  T cr = rw ? rsqrt_(h_r * GRAVITY) : Z;
-  T un_l = lw ? qn_l / h_l : Z;
-  T un_r = rw ? qn_r / h_r : Z;
-  T ut_l = lw ? qt_l / h_l : Z;
-  T ut_r = rw ? qt_r / h_r : Z;
+  T cl = lw ? rsqrt_(h_l * R(GRAVITY)) : Z;
+  T cr = rw ? rsqrt_(h_r * R(GRAVITY)) : Z;
+  T un_l = lw ? div(qn_l, h_l) : Z;
+  T un_r = rw ? div(qn_r, h_r) : Z;
+  T ut_l = lw ? div(qt_l, h_l) : Z;
+  T ut_r = rw ? div(qt_r, h_r) : Z;
   T sl, sr;
   if (lw && rw) {
     sl = rmin(un_l - cl, un_r - cr);
     sr = rmax(un_l + cl, un_r + cr);
   } else if (rw) {
-    sl = un_r - cr * 2.0;
+    sl = un_r - cr * R(2.0);
     sr = un_r + cr;
   } else {
     sl = un_l - cl;
-    sr = un_l + cl * 2.0;
+    sr = un_l + cl * R(2.0);
   }
-  if (val(sl) >= 0.0) {
+  if (val(sl) >= R(0)) {
     if (lw) {
       *fm = qn_l;
-      *fn = qn_l * un_l + h_l * h_l * (0.5 * GRAVITY);
+      *fn = qn_l * un_l + h_l * h_l * R(0.5 * GRAVITY);
       *ft = qn_l * ut_l;
     } else {
       *fm = Z;
@@ -210,10 +249,10 @@ HF_HD void hllc_normal_flux(T h_l, T qn_l, T qt_l, T h_r, T qn_r, T qt_r, T* fm,
     }
     return;
   }
-  if (val(sr) <= 0.0) {
+  if (val(sr) <= R(0)) {
     if (rw) {
       *fm = qn_r;
-      *fn = qn_r * un_r + h_r * h_r * (0.5 * GRAVITY);
+      *fn = qn_r * un_r + h_r * h_r * R(0.5 * GRAVITY);
       *ft = qn_r * ut_r;
     } else {
       *fm = Z;
@@ -223,32 +262,30 @@ HF_HD void hllc_normal_flux(T h_l, T qn_l, T qt_l, T h_r, T qn_r, T qt_r, T* fm,
     return;
   }
   T denom = h_r * (un_r - sr) - h_l * (un_l - sl);
-  T s_star = (__builtin_fabs(val(denom)) > 1.0e-12)
-                 ? (h_r * (un_r - sr) * sl - h_l * (un_l - sl) * sr) / denom
-                 : (sl + sr) * 0.5;
-  if (val(s_star) >= 0.0) {
-    T fm_l = Z, fn_l = Z, ft_l = Z;
-    if (lw) {
-      fm_l = qn_l;
-      fn_l = qn_l * un_l + h_l * h_l * (0.5 * GRAVITY);
-      ft_l = qn_l * ut_l;
-    }
-    T h_star_l = h_l * (sl - un_l) / (sl - s_star);
-    *fm = fm_l + (h_star_l - h_l) * sl;
-    *fn = fn_l + (h_star_l * s_star - qn_l) * sl;
-    *ft = ft_l + (h_star_l * ut_l - qt_l) * sl;
-  } else {
-    T fm_r = Z, fn_r = Z, ft_r = Z;
-    if (rw) {
-      fm_r = qn_r;
-      fn_r = qn_r * un_r + h_r * h_r * (0.5 * GRAVITY);
-      ft_r = qn_r * ut_r;
-    }
-    T h_star_r = h_r * (sr - un_r) / (sr - s_star);
-    *fm = fm_r + (h_star_r - h_r) * sr;
-    *fn = fn_r + (h_star_r * s_star - qn_r) * sr;
-    *ft = ft_r + (h_star_r * ut_r - qt_r) * sr;
+  T s_star = (rabs(val(denom)) > R(1.0e-12))
+                 ? div(h_r * (un_r - sr) * sl - h_l * (un_l - sl) * sr, denom)
+                 : (sl + sr) * R(0.5);
+  // HLLC sampling (riemann.rs:191-203): the same expressions on the side
+  // the contact wave leaves upwind, with that side's operands selected
+  // first so a wavefront evaluates one star state instead of both branches.
+  const bool left = val(s_star) >= R(0);
+  const bool wK = left ? lw : rw;
+  const T hK = left ? h_l : h_r;
+  const T qnK = left ? qn_l : qn_r;
+  const T qtK = left ? qt_l : qt_r;
+  const T unK = left ? un_l : un_r;
+  const T utK = left ? ut_l : ut_r;
+  const T sK = left ? sl : sr;
+  T fmK = Z, fnK = Z, ftK = Z;
+  if (wK) {
+    fmK = qnK;
+    fnK = qnK * unK + hK * hK * R(0.5 * GRAVITY);
+    ftK = qnK * utK;
   }
+  T h_star = div(hK * (sK - unK), sK - s_star);
+  *fm = fmK + (h_star - hK) * sK;
+  *fn = fnK + (h_star * s_star - qnK) * sK;
+  *ft = ftK + (h_star * utK - qtK) * sK;
 }
 
 // update.rs:256-353 with z_left = z_right = z_face. Outputs the shared
@@ -257,23 +294,24 @@ HF_HD void hllc_normal_flux(T h_l, T qn_l, T qt_l, T h_r, T qn_r, T qt_r, T* fm,
 template <class T>
 HF_HD void well_balanced_face(T lh, T lqn, T lqt, T rh, T rqn, T rqt, T zf, T* mass, T* nm_minus,
                               T* nm_plus, T* tang) {
+  using R = typename ScalarOf<T>::type;
   const T Z = cst<T>(0.0);
   T z_max = rmax(zf, zf);
   T hsl = rmax(lh + zf - z_max, Z);
   T hsr = rmax(rh + zf - z_max, Z);
   T ul = Z, vl = Z, ur = Z, vr = Z;
-  if (val(lh) > H_VEL) {
-    ul = lqn / lh;
-    vl = lqt / lh;
+  if (val(lh) > R(H_VEL)) {
+    ul = div(lqn, lh);
+    vl = div(lqt, lh);
   }
-  if (val(rh) > H_VEL) {
-    ur = rqn / rh;
-    vr = rqt / rh;
+  if (val(rh) > R(H_VEL)) {
+    ur = div(rqn, rh);
+    vr = div(rqt, rh);
   }
   T fm, fn, ft;
   hllc_normal_flux(hsl, hsl * ul, hsl * vl, hsr, hsr * ur, hsr * vr, &fm, &fn, &ft);
-  T corr_l = (lh * lh - hsl * hsl) * (0.5 * GRAVITY);
-  T corr_r = (rh * rh - hsr * hsr) * (0.5 * GRAVITY);
+  T corr_l = (lh * lh - hsl * hsl) * R(0.5 * GRAVITY);
+  T corr_r = (rh * rh - hsr * hsr) * R(0.5 * GRAVITY);
   *mass = fm;
   *nm_minus = fn + corr_l;
   *nm_plus = fn + corr_r;
@@ -337,6 +375,7 @@ template <class T, int TW, int TH> struct TileSmem {
 // ------------------------------------------------------------ the stage
 template <class T, int TW, int TH> struct Stage {
   using S = TileSmem<T, TW, TH>;
+  using R = typename ScalarOf<T>::type;
 
   // Face bed of x-face fj in local row li (update.rs:676-700).
   static HF_HD T zface_x(const S& s, const StageArgs<T>& a, int64_t gi0, int64_t gj0, int li, int fj) {
@@ -344,12 +383,12 @@ template <class T, int TW, int TH> struct Stage {
     if (gjf == 0) {
       T zc = s.z[S::i3(li, fj)];
       T zn = (a.nc >= 2) ? s.z[S::i3(li, fj + 1)] : zc;
-      return (ghost_bed(a.bc_kind[SIDE_W], zc, zn) + zc) * 0.5;
+      return (ghost_bed(a.bc_kind[SIDE_W], zc, zn) + zc) * R(0.5);
     }
     if (gjf == a.nc) {
       T zc = s.z[S::i3(li, fj - 1)];
       T zn = (a.nc >= 2) ? s.z[S::i3(li, fj - 2)] : zc;
-      return (zc + ghost_bed(a.bc_kind[SIDE_E], zc, zn)) * 0.5;
+      return (zc + ghost_bed(a.bc_kind[SIDE_E], zc, zn)) * R(0.5);
     }
     return interior_z_face(s.dry[S::i4(li, fj - 1)] != 0, s.z[S::i3(li, fj - 1)],
                            s.dry[S::i4(li, fj)] != 0, s.z[S::i3(li, fj)]);
@@ -360,12 +399,12 @@ template <class T, int TW, int TH> struct Stage {
     if (gif == 0) {
       T zc = s.z[S::i3(fi, lj)];
       T zn = (a.nr >= 2) ? s.z[S::i3(fi + 1, lj)] : zc;
-      return (ghost_bed(a.bc_kind[SIDE_N], zc, zn) + zc) * 0.5;
+      return (ghost_bed(a.bc_kind[SIDE_N], zc, zn) + zc) * R(0.5);
     }
     if (gif == a.nr) {
       T zc = s.z[S::i3(fi - 1, lj)];
       T zn = (a.nr >= 2) ? s.z[S::i3(fi - 2, lj)] : zc;
-      return (zc + ghost_bed(a.bc_kind[SIDE_S], zc, zn)) * 0.5;
+      return (zc + ghost_bed(a.bc_kind[SIDE_S], zc, zn)) * R(0.5);
     }
     return interior_z_face(s.dry[S::i4(fi - 1, lj)] != 0, s.z[S::i3(fi - 1, lj)],
                            s.dry[S::i4(fi, lj)] != 0, s.z[S::i3(fi, lj)]);
@@ -377,6 +416,9 @@ template <class T, int TW, int TH> struct Stage {
   // reference does.
   static HF_HD void phase_load(S& s, const StageArgs<T>& a, int64_t gi0, int64_t gj0, int tid, int nt) {
     const T Z = cst<T>(0.0);
+    const int64_t ci = gi0 + TH / 2 < a.nr ? gi0 + TH / 2 : a.nr - 1;
+    const int64_t cj = gj0 + TW / 2 < a.nc ? gj0 + TW / 2 : a.nc - 1;
+    const double tdat = tile_datum(a, ci, cj);
     for (int k = tid; k < S::N4; k += nt) {
       const int li = k / S::W4 - 4, lj = k % S::W4 - 4;
       const int64_t gi = gi0 + li, gj = gj0 + lj;
@@ -385,14 +427,14 @@ template <class T, int TW, int TH> struct Stage {
       if (inside) {
         const int64_t idx = gi * a.nc + gj;
         T hh = a.ih.ld(idx);
-        s.dry[k] = val(hh) <= H_DRY;
+        s.dry[k] = val(hh) <= R(H_DRY);
         if (in3) {
           const int k3 = S::i3(li, lj);
           s.h[k3] = hh;
-          s.z[k3] = a.z.ld(idx);
-          if (val(hh) > H_VEL) {
-            s.u[k3] = a.ihu.ld(idx) / hh;
-            s.v[k3] = a.ihv.ld(idx) / hh;
+          s.z[k3] = load_bed(a, gi, gj, idx, tdat);
+          if (val(hh) > R(H_VEL)) {
+            s.u[k3] = div(a.ihu.ld(idx), hh);
+            s.v[k3] = div(a.ihv.ld(idx), hh);
           } else {
             s.u[k3] = Z;
             s.v[k3] = Z;
@@ -417,7 +459,7 @@ template <class T, int TW, int TH> struct Stage {
   // the near-dry flags of update.rs:433-444 (window clipped to the domain).
   static HF_HD void phase_diffs(S& s, const StageArgs<T>& a, int64_t gi0, int64_t gj0, int tid, int nt) {
     const T Z = cst<T>(0.0);
-    const double dx = a.dx, dy = a.dy;
+    const R dx = R(a.dx), dy = R(a.dy);
     const int n_total = S::NDX + S::NDY + S::NND;
     for (int k = tid; k < n_total; k += nt) {
       if (k < S::NDX) {
@@ -426,9 +468,9 @@ template <class T, int TW, int TH> struct Stage {
         T de = Z, du = Z, dv = Z;
         if (gi >= 0 && gi < a.nr && gj >= 1 && gj < a.nc) {
           const int l = S::i3(li, fj - 1), r = S::i3(li, fj);
-          de = (s.h[r] + s.z[r] - (s.h[l] + s.z[l])) / dx;
-          du = (s.u[r] - s.u[l]) / dx;
-          dv = (s.v[r] - s.v[l]) / dx;
+          de = div(s.h[r] + s.z[r] - (s.h[l] + s.z[l]), dx);
+          du = div(s.u[r] - s.u[l], dx);
+          dv = div(s.v[r] - s.v[l], dx);
         }
         s.dxe[k] = de;
         s.dxu[k] = du;
@@ -440,9 +482,9 @@ template <class T, int TW, int TH> struct Stage {
         T de = Z, du = Z, dv = Z;
         if (gj >= 0 && gj < a.nc && gi >= 1 && gi < a.nr) {
           const int t = S::i3(fi - 1, lj), b = S::i3(fi, lj);
-          de = (s.h[b] + s.z[b] - (s.h[t] + s.z[t])) / dy;
-          du = (s.u[b] - s.u[t]) / dy;
-          dv = (s.v[b] - s.v[t]) / dy;
+          de = div(s.h[b] + s.z[b] - (s.h[t] + s.z[t]), dy);
+          du = div(s.u[b] - s.u[t], dy);
+          dv = div(s.v[b] - s.v[t], dy);
         }
         s.dye[kk] = de;
         s.dyu[kk] = du;
@@ -561,7 +603,7 @@ template <class T, int TW, int TH> struct Stage {
         else
           well_balanced_face(h, hu, hv, gh, ghu, ghv, zf, &m, &na, &nb, &tg);
       } else if (!(s.dry[S::i4(li, fj - 1)] && s.dry[S::i4(li, fj)])) {
-        const double half_dx = 0.5 * a.dx;
+        const R half_dx = R(0.5 * a.dx);
         const int l = S::i3(li, fj - 1), r = S::i3(li, fj);
         T sle, slu, slv, sre, sru, srv;
         slope_x(s, a, li, fj - 1, gjf - 1, &sle, &slu, &slv);
@@ -609,7 +651,7 @@ template <class T, int TW, int TH> struct Stage {
         else
           well_balanced_face(h, hv, hu, gh, ghv, ghu, zf, &m, &na, &nb, &tg);
       } else if (!(s.dry[S::i4(fi - 1, lj)] && s.dry[S::i4(fi, lj)])) {
-        const double half_dy = 0.5 * a.dy;
+        const R half_dy = R(0.5 * a.dy);
         const int t = S::i3(fi - 1, lj), b = S::i3(fi, lj);
         T ste, stu, stv, sbe, sbu, sbv;
         slope_y(s, a, fi - 1, lj, gif - 1, &ste, &stu, &stv);
@@ -646,7 +688,7 @@ template <class T, int TW, int TH> struct Stage {
   // (update.rs:953-982).
   static HF_HD void phase_alpha(S& s, const StageArgs<T>& a, int64_t gi0, int64_t gj0, int tid, int nt,
                                 double dt) {
-    const double dt_dx = dt / a.dx, dt_dy = dt / a.dy;
+    const R dt_dx = R(dt / a.dx), dt_dy = R(dt / a.dy);
     const T Z = cst<T>(0.0);
     for (int k = tid; k < S::NA; k += nt) {
       const int li = k / S::WA - 1, lj = k % S::WA - 1;
@@ -658,12 +700,12 @@ template <class T, int TW, int TH> struct Stage {
         T fy_bottom = s.fym[S::ify(li + 1, lj)];
         T fy_top = s.fym[S::ify(li, lj)];
         T out = Z;
-        if (val(fx_right) > 0.0) out = out + fx_right * dt_dx;
-        if (val(fx_left) < 0.0) out = out + (-fx_left) * dt_dx;
-        if (val(fy_bottom) > 0.0) out = out + fy_bottom * dt_dy;
-        if (val(fy_top) < 0.0) out = out + (-fy_top) * dt_dy;
-        T avail = rmax(s.h[S::i3(li, lj)] - H_DRY, Z);
-        if (val(out) > val(avail) && val(out) > 0.0) al = avail / out;
+        if (val(fx_right) > R(0)) out = out + fx_right * dt_dx;
+        if (val(fx_left) < R(0)) out = out + (-fx_left) * dt_dx;
+        if (val(fy_bottom) > R(0)) out = out + fy_bottom * dt_dy;
+        if (val(fy_top) < R(0)) out = out + (-fy_top) * dt_dy;
+        T avail = rmax(s.h[S::i3(li, lj)] - R(H_DRY), Z);
+        if (val(out) > val(avail) && val(out) > R(0)) al = div(avail, out);
       }
       s.alpha[k] = al;
     }
@@ -682,10 +724,10 @@ template <class T, int TW, int TH> struct Stage {
     if (gjf > 0 && gjf < a.nc && s.dry[S::i4(li, fj - 1)] && s.dry[S::i4(li, fj)]) return;
     T al;
     if (gjf == 0)
-      al = val(*m) < 0.0 ? s.alpha[S::ia(li, fj)] : cst<T>(1.0);
+      al = val(*m) < R(0) ? s.alpha[S::ia(li, fj)] : cst<T>(1.0);
     else if (gjf == a.nc)
-      al = val(*m) > 0.0 ? s.alpha[S::ia(li, fj - 1)] : cst<T>(1.0);
-    else if (val(*m) > 0.0)
+      al = val(*m) > R(0) ? s.alpha[S::ia(li, fj - 1)] : cst<T>(1.0);
+    else if (val(*m) > R(0))
       al = s.alpha[S::ia(li, fj - 1)];
     else
       al = s.alpha[S::ia(li, fj)];
@@ -706,10 +748,10 @@ template <class T, int TW, int TH> struct Stage {
     if (gif > 0 && gif < a.nr && s.dry[S::i4(fi - 1, lj)] && s.dry[S::i4(fi, lj)]) return;
     T al;
     if (gif == 0)
-      al = val(*m) < 0.0 ? s.alpha[S::ia(fi, lj)] : cst<T>(1.0);
+      al = val(*m) < R(0) ? s.alpha[S::ia(fi, lj)] : cst<T>(1.0);
     else if (gif == a.nr)
-      al = val(*m) > 0.0 ? s.alpha[S::ia(fi - 1, lj)] : cst<T>(1.0);
-    else if (val(*m) > 0.0)
+      al = val(*m) > R(0) ? s.alpha[S::ia(fi - 1, lj)] : cst<T>(1.0);
+    else if (val(*m) > R(0))
       al = s.alpha[S::ia(fi - 1, lj)];
     else
       al = s.alpha[S::ia(fi, lj)];
@@ -723,16 +765,16 @@ template <class T, int TW, int TH> struct Stage {
   // domain edges, of its ghosts (update.rs:179-201, no H_VEL cut there).
   static HF_HD void cell_wave_speed(const StageArgs<T>& a, int64_t gi, int64_t gj, T h, T hu, T hv, T zc,
                                     const T* zn4, const T* man, double* sx, double* sy) {
-    const double hv_ = val(h);
-    if (hv_ > H_DRY) {
-      double c = __builtin_sqrt(GRAVITY * hv_);
-      double u = 0.0, v = 0.0;
-      if (hv_ > H_VEL) {
-        u = val(hu) / hv_;
-        v = val(hv) / hv_;
+    const R hv_ = val(h);
+    if (hv_ > R(H_DRY)) {
+      R c = rsqrt_(R(GRAVITY) * hv_);
+      R u = R(0), v = R(0);
+      if (hv_ > R(H_VEL)) {
+        u = div(val(hu), hv_);
+        v = div(val(hv), hv_);
       }
-      *sx = rmax(*sx, __builtin_fabs(u) + c);
-      *sy = rmax(*sy, __builtin_fabs(v) + c);
+      *sx = rmax(*sx, (double)(rabs(u) + c));
+      *sy = rmax(*sy, (double)(rabs(v) + c));
     }
     if (!a.cfl_with_bcs) return;
     const bool edge[4] = {gi == 0, gi == a.nr - 1, gj == 0, gj == a.nc - 1};
@@ -743,13 +785,13 @@ template <class T, int TW, int TH> struct Stage {
       T gh, ghu, ghv;
       ghost_state(a.bc_kind[side], a.bc_param[side], side, h, hu, hv, zc, zn4[side], have_next,
                   xf ? a.dx : a.dy, *man, &gh, &ghu, &ghv);
-      const double g = val(gh);
-      if (g <= H_DRY) continue;
-      double c = __builtin_sqrt(GRAVITY * g);
-      double u = val(ghu) / g;
-      double v = val(ghv) / g;
-      *sx = rmax(*sx, __builtin_fabs(u) + c);
-      *sy = rmax(*sy, __builtin_fabs(v) + c);
+      const R g = val(gh);
+      if (g <= R(H_DRY)) continue;
+      R c = rsqrt_(R(GRAVITY) * g);
+      R u = div(val(ghu), g);
+      R v = div(val(ghv), g);
+      *sx = rmax(*sx, (double)(rabs(u) + c));
+      *sy = rmax(*sy, (double)(rabs(v) + c));
     }
   }
 
@@ -759,20 +801,20 @@ template <class T, int TW, int TH> struct Stage {
   // (source.rs:52-69).
   static HF_HD void forcing(const StageArgs<T>& a, int64_t gi, int64_t gj, double dt, T* h, T* hu, T* hv) {
     const T Z = cst<T>(0.0);
-    if (a.friction && val(*h) > a.friction_dry_tol) {
+    if (a.friction && val(*h) > R(a.friction_dry_tol)) {
       T n = a.man.ld(gi * a.nc + gj);
-      if (val(n) != 0.0) {
-        const double dt_g = dt * GRAVITY;
-        T speed = rsqrt_(*hu * *hu + *hv * *hv) / *h;
-        T factor = n * n * speed / pow43(*h) * dt_g + 1.0;
-        *hu = *hu / factor;
-        *hv = *hv / factor;
+      if (val(n) != R(0)) {
+        const R dt_g = R(dt * GRAVITY);
+        T speed = div(rsqrt_(*hu * *hu + *hv * *hv), *h);
+        T factor = div(n * n * speed, pow43(*h)) * dt_g + R(1.0);
+        *hu = div(*hu, factor);
+        *hv = div(*hv, factor);
       }
     }
     if (a.rain_rate != 0.0) {
-      const double dh = a.rain_rate * dt;
+      const R dh = R(a.rain_rate * dt);
       T nh = *h + dh;
-      if (val(nh) <= 0.0) {
+      if (val(nh) <= R(0)) {
         *h = Z;
         *hu = Z;
         *hv = Z;
@@ -784,9 +826,9 @@ template <class T, int TW, int TH> struct Stage {
       const double cell_area = a.dx * a.dy;
       for (int k = 0; k < a.n_src; ++k) {
         if (a.srcs[k].row != gi || a.srcs[k].col != gj) continue;
-        const double dh = a.srcs[k].q * dt / cell_area;
+        const R dh = R(a.srcs[k].q * dt / cell_area);
         *h = rmax(*h + dh, Z);
-        if (val(*h) == 0.0) {
+        if (val(*h) == R(0)) {
           *hu = Z;
           *hv = Z;
         }
@@ -801,7 +843,7 @@ template <class T, int TW, int TH> struct Stage {
   static HF_HD void phase_update(S& s, const StageArgs<T>& a, int64_t gi0, int64_t gj0, int tid, int nt,
                                  double dt, double* wsx, double* wsy) {
     const T Z = cst<T>(0.0);
-    const double dt_dx = dt / a.dx, dt_dy = dt / a.dy;
+    const R dt_dx = R(dt / a.dx), dt_dy = R(dt / a.dy), rdt = R(dt);
     for (int k = tid; k < TW * TH; k += nt) {
       const int li = k / TW, lj = k % TW;
       const int64_t gi = gi0 + li, gj = gj0 + lj;
@@ -830,44 +872,44 @@ template <class T, int TW, int TH> struct Stage {
 
         T eta_cell = h_old + s.z[c3];
         T sex = slope_x_eta(s, a, li, lj, gj);
-        T eta_left = (gj > 0) ? eta_cell - sex * (0.5 * a.dx) : eta_cell;
-        T eta_right = (gj + 1 < a.nc) ? eta_cell + sex * (0.5 * a.dx) : eta_cell;
+        T eta_left = (gj > 0) ? eta_cell - sex * R(0.5 * a.dx) : eta_cell;
+        T eta_right = (gj + 1 < a.nc) ? eta_cell + sex * R(0.5 * a.dx) : eta_cell;
         T hfl = rmax(eta_left - zface_x(s, a, gi0, gj0, li, lj), Z);
         T hfr = rmax(eta_right - zface_x(s, a, gi0, gj0, li, lj + 1), Z);
         T zel = eta_left - hfl;
         T zer = eta_right - hfr;
-        T s_hu = (hfl + hfr) * (zel - zer) * (0.5 * GRAVITY) / a.dx;
+        T s_hu = div((hfl + hfr) * (zel - zer) * R(0.5 * GRAVITY), R(a.dx));
 
         T sey = slope_y_eta(s, a, li, lj, gi);
-        T eta_top = (gi > 0) ? eta_cell - sey * (0.5 * a.dy) : eta_cell;
-        T eta_bot = (gi + 1 < a.nr) ? eta_cell + sey * (0.5 * a.dy) : eta_cell;
+        T eta_top = (gi > 0) ? eta_cell - sey * R(0.5 * a.dy) : eta_cell;
+        T eta_bot = (gi + 1 < a.nr) ? eta_cell + sey * R(0.5 * a.dy) : eta_cell;
         T hft = rmax(eta_top - zface_y(s, a, gi0, gj0, li, lj), Z);
         T hfb = rmax(eta_bot - zface_y(s, a, gi0, gj0, li + 1, lj), Z);
         T zet = eta_top - hft;
         T zeb = eta_bot - hfb;
-        T s_hv = (hft + hfb) * (zet - zeb) * (0.5 * GRAVITY) / a.dy;
+        T s_hv = div((hft + hfb) * (zet - zeb) * R(0.5 * GRAVITY), R(a.dy));
 
-        if (val(new_h) <= H_VEL) {
+        if (val(new_h) <= R(H_VEL)) {
           nh = rmax(new_h, Z);
           nhu = Z;
           nhv = Z;
         } else {
           T hu_old = Z, hv_old = Z;
-          if (val(h_old) > H_VEL) {
+          if (val(h_old) > R(H_VEL)) {
             hu_old = a.ihu.ld(idx);
             hv_old = a.ihv.ld(idx);
           }
           nh = new_h;
-          nhu = hu_old - dhu + s_hu * dt;
-          nhv = hv_old - dhv + s_hv * dt;
+          nhu = hu_old - dhu + s_hu * rdt;
+          nhv = hv_old - dhv + s_hv * rdt;
         }
       }
       if (MODE == MODE_FINAL_RK2) {
         T ph = a.nh.ld(idx), phu = a.nhu.ld(idx), phv = a.nhv.ld(idx);
-        nh = (ph + nh) * 0.5;
-        nhu = (phu + nhu) * 0.5;
-        nhv = (phv + nhv) * 0.5;
-        if (val(nh) <= H_VEL) {
+        nh = (ph + nh) * R(0.5);
+        nhu = (phu + nhu) * R(0.5);
+        nhv = (phv + nhv) * R(0.5);
+        if (val(nh) <= R(H_VEL)) {
           nh = rmax(nh, Z);
           nhu = Z;
           nhv = Z;

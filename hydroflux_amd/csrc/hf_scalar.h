@@ -1,13 +1,19 @@
-// Scalar types for the stage kernels: double and a forward-mode dual.
+// Scalar types for the stage kernels: double, a forward-mode dual, and
+// float for the FP32 precision form.
 //
 // The dual follows the reference's rules exactly (autograd/src/dual.rs:12-299,
 // real.rs:115-149): quotient rule with (a'b - ab')/(b*b), sqrt(0) has tangent
 // 0, max/min ties average the tangents, powf(v, n) has tangent
 // n * v^(n-1) * v'. Every branch decision is taken on the value alone.
 //
+// Each value type T has a scalar type R = ScalarOf<T>::type (double for
+// double and dual, float for float); constants, thresholds, dt and the grid
+// spacings enter T arithmetic as R. For double and dual this is the same
+// IEEE double arithmetic as writing the literals directly.
+//
 // Shared by the device code (hipcc) and the CPU build of the same tile code
-// (g++), so it uses only IEEE operations and the deterministic powers of
-// hf_pow.h.
+// (g++), so the double and dual paths use only IEEE operations and the
+// deterministic powers of hf_pow.h.
 #pragma once
 #include <math.h>
 
@@ -19,12 +25,24 @@ struct dual {
   double v, d;
 };
 
+template <class T> struct ScalarOf {
+  using type = double;
+};
+template <> struct ScalarOf<float> {
+  using type = float;
+};
+
 HF_HD double val(double x) { return x; }
 HF_HD double val(dual x) { return x.v; }
+HF_HD float val(float x) { return x; }
+
+HF_HD double rabs(double x) { return __builtin_fabs(x); }
+HF_HD float rabs(float x) { return __builtin_fabsf(x); }
 
 template <class T> HF_HD T cst(double x);
 template <> HF_HD double cst<double>(double x) { return x; }
 template <> HF_HD dual cst<dual>(double x) { return dual{x, 0.0}; }
+template <> HF_HD float cst<float>(double x) { return (float)x; }
 
 HF_HD dual operator+(dual a, dual b) { return {a.v + b.v, a.d + b.d}; }
 HF_HD dual operator-(dual a, dual b) { return {a.v - b.v, a.d - b.d}; }
@@ -36,9 +54,25 @@ HF_HD dual operator-(dual a, double b) { return {a.v - b, a.d}; }
 HF_HD dual operator*(dual a, double b) { return {a.v * b, a.d * b}; }
 HF_HD dual operator/(dual a, double b) { return {a.v / b, a.d / b}; }
 
+// Division. double and dual: IEEE division, as the reference. float (the
+// precision form): reciprocal then multiply, ~1 ulp, a fraction of the
+// correctly rounded sequence's cost.
+HF_HD double div(double a, double b) { return a / b; }
+HF_HD dual div(dual a, dual b) { return a / b; }
+HF_HD dual div(dual a, double b) { return a / b; }
+#ifdef __HIP_DEVICE_COMPILE__
+HF_HD float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+HF_HD float fsqrt_(float a) { return __builtin_amdgcn_sqrtf(a); }
+#else
+HF_HD float div(float a, float b) { return a / b; }
+HF_HD float fsqrt_(float a) { return sqrtf(a); }
+#endif
+
 // f64::max / f64::min ignore a NaN operand; IEEE maxNum/minNum do the same.
 HF_HD double rmax(double a, double b) { return __builtin_fmax(a, b); }
 HF_HD double rmin(double a, double b) { return __builtin_fmin(a, b); }
+HF_HD float rmax(float a, float b) { return __builtin_fmaxf(a, b); }
+HF_HD float rmin(float a, float b) { return __builtin_fminf(a, b); }
 HF_HD dual rmax(dual a, dual b) {
   if (a.v > b.v) return a;
   if (b.v > a.v) return b;
@@ -51,6 +85,7 @@ HF_HD dual rmin(dual a, dual b) {
 }
 
 HF_HD double rsqrt_(double a) { return __builtin_sqrt(a); }
+HF_HD float rsqrt_(float a) { return fsqrt_(a); }
 HF_HD dual rsqrt_(dual a) {
   double s = __builtin_sqrt(a.v);
   double d = (s == 0.0) ? 0.0 : a.d / (2.0 * s);
@@ -63,10 +98,12 @@ HF_HD dual pow43(dual x) {
   const double n = 4.0 / 3.0;
   return {hfm::pow_4_3(x.v), n * hfm::pow_1_3m(x.v) * x.d};
 }
+HF_HD float pow43(float x) { return x * cbrtf(x); }
 
 // General x.powf(n) for the Discharge-on-dry ghost depth (boundary.rs:212),
 // the only other power on the path.
 HF_HD double powf_gen(double x, double n) { return pow(x, n); }
 HF_HD dual powf_gen(dual x, double n) { return {pow(x.v, n), n * pow(x.v, n - 1.0) * x.d}; }
+HF_HD float powf_gen(float x, double n) { return powf(x, (float)n); }
 
 }  // namespace hf

@@ -70,6 +70,7 @@ def voronoi_classes(n_rows, n_cols, n_sites=64, seed=SEED_LAND):
     sites = rng.uniform([0, 0], [n_rows, n_cols], size=(n_sites, 2))
     cls = rng.integers(0, len(LAND_COVER_N), size=n_sites)
     tree = cKDTree(sites)
+    workers = -1
     out = np.empty((n_rows, n_cols), np.int8)
     rows = np.arange(n_rows) + 0.5
     cols = np.arange(n_cols) + 0.5
@@ -77,7 +78,7 @@ def voronoi_classes(n_rows, n_cols, n_sites=64, seed=SEED_LAND):
     for r0 in range(0, n_rows, step):
         rr = rows[r0:r0 + step]
         pts = np.stack(np.meshgrid(rr, cols, indexing="ij"), -1).reshape(-1, 2)
-        _, idx = tree.query(pts)
+        _, idx = tree.query(pts, workers=workers)
         out[r0:r0 + step] = cls[idx].reshape(len(rr), n_cols)
     return out
 
@@ -97,3 +98,26 @@ def flood_scenario(n=1024, dx=5.0, offset=0.0, rain_mm_h=50.0, inflow=100.0, val
     return FloodScenario(n_rows, n_cols, dx, z, np.ascontiguousarray(man), land,
                          rain_rate=rain_mm_h * 1e-3 / 3600.0,
                          sources=[(n_rows // 2, 0, inflow)] if inflow else [])
+
+
+def scenario_mix(n=256, seed=0x48464403, min_side=128, max_side=2048, rain_range=(10.0, 100.0),
+                 duration_range=None, dx=5.0):
+    """The batching workload: `n` scenarios, long side log-uniform in
+    [min_side, max_side], aspect ratio U(1, 4) (short side = max(min_side,
+    long / aspect)), each with its own seeded DEM and land cover, rain
+    U(10, 100) mm/h, an inflow at its upstream valley cell, and (optionally)
+    a duration U(duration_range) simulated seconds."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n):
+        s = int(round(np.exp(rng.uniform(np.log(min_side), np.log(max_side)))))
+        a = rng.uniform(1.0, 4.0)
+        short = max(min_side, int(round(s / a)))
+        rows, cols = (short, s) if rng.random() < 0.5 else (s, short)
+        rain = rng.uniform(*rain_range)
+        sc = flood_scenario(n=rows, n_cols=cols, dx=dx, rain_mm_h=rain, inflow=100.0,
+                            seed_dem=seed * 1000 + 2 * k, seed_land=seed * 1000 + 2 * k + 1)
+        if duration_range is not None:
+            sc.t_end = float(rng.uniform(*duration_range))
+        out.append(sc)
+    return out

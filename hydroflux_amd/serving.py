@@ -1,0 +1,153 @@
+"""Continuous-batching serving loop for flood-scenario requests on one GPU.
+
+Requests arrive over time (e.g. Poisson). Each is admitted into the resident
+batch of its kind (FP32 precision form for plain requests, Dual for requests
+carrying forward-mode tangents) as soon as a slot and enough HBM are free;
+its state stays on the device until its clock reaches its end time. The GPU
+advances every resident scenario together in chunks of steps; between
+chunks the host retires finished scenarios (download, free) and admits
+waiting ones, so new work overlaps with work already on the device.
+
+Reported: per-request latency-to-completion (completion - arrival, wall
+clock), its p50/p99, and the aggregate cell-updates/s over the run.
+"""
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional
+
+import numpy as np
+
+from .batch import Batch, BatchFull, ScenarioSpec, footprint
+
+
+@dataclass
+class Request:
+    rid: int
+    arrival: float          # seconds after the start of the run
+    spec: ScenarioSpec
+    kind: str = "f32"       # "f32" or "dual"
+    # filled by the server
+    admitted: Optional[float] = None
+    completed: Optional[float] = None
+    steps: int = 0
+    sim_time: float = 0.0
+    status: int = 0
+    result: Optional[np.ndarray] = None
+    tangent: Optional[np.ndarray] = None
+
+    @property
+    def latency(self):
+        return None if self.completed is None else self.completed - self.arrival
+
+
+@dataclass
+class ServeReport:
+    requests: List[Request]
+    wall_s: float
+    cell_updates: float
+    latencies: np.ndarray = field(default=None)
+
+    @property
+    def cell_updates_per_s(self):
+        return self.cell_updates / self.wall_s
+
+    def percentile(self, q):
+        return float(np.percentile(self.latencies, q))
+
+    def summary(self):
+        return {
+            "requests": len(self.requests),
+            "wall_s": self.wall_s,
+            "cell_updates_per_s": self.cell_updates_per_s,
+            "latency_p50_s": self.percentile(50),
+            "latency_p99_s": self.percentile(99),
+            "latency_mean_s": float(np.mean(self.latencies)),
+            "failed": int(sum(1 for r in self.requests if r.status != 0)),
+        }
+
+
+def poisson_arrivals(n, rate, seed=0):
+    """n arrival times of a Poisson process with `rate` requests/s (rate=inf: all at 0)."""
+    if not np.isfinite(rate):
+        return np.zeros(n)
+    rng = np.random.default_rng(seed)
+    return np.cumsum(rng.exponential(1.0 / rate, size=n))
+
+
+class Server:
+    def __init__(self, capacity_bytes: Dict[str, int], max_slots: int = 512, chunk_steps: int = 32,
+                 keep_results: bool = True, autotune_steps: int = 0, tile_config: Optional[Dict[str, int]] = None):
+        self.batches = {k: Batch(k, capacity_bytes=v, max_slots=max_slots) for k, v in capacity_bytes.items()}
+        self.chunk = int(chunk_steps)
+        self.keep = keep_results
+        self.autotune_steps = autotune_steps
+        for k, ci in (tile_config or {}).items():
+            self.batches[k].set_config(ci)
+
+    def close(self):
+        for b in self.batches.values():
+            b.close()
+
+    def run(self, requests: List[Request]) -> ServeReport:
+        pending = sorted(requests, key=lambda r: r.arrival)
+        waiting: Dict[str, List[Request]] = {k: [] for k in self.batches}
+        resident: Dict[str, Dict[int, Request]] = {k: {} for k in self.batches}
+        cell_updates = 0.0
+        i = 0
+        t0 = time.perf_counter()
+        tuned = set()
+        while i < len(pending) or any(waiting[k] or resident[k] for k in self.batches):
+            now = time.perf_counter() - t0
+            while i < len(pending) and pending[i].arrival <= now:
+                waiting[pending[i].kind].append(pending[i])
+                i += 1
+            # admission by HBM footprint, FIFO per kind
+            for k, q in waiting.items():
+                while q:
+                    r = q[0]
+                    try:
+                        slot = self.batches[k].add(r.spec)
+                    except BatchFull:
+                        break
+                    r.admitted = time.perf_counter() - t0
+                    resident[k][slot] = r
+                    q.pop(0)
+            active = [k for k in self.batches if resident[k]]
+            if not active:
+                if i < len(pending):  # idle until the next arrival
+                    time.sleep(max(0.0, min(0.01, pending[i].arrival - (time.perf_counter() - t0))))
+                continue
+            for k in active:
+                if self.autotune_steps and k not in tuned and len(resident[k]) > 1:
+                    self.batches[k].autotune(self.autotune_steps)
+                    tuned.add(k)
+                self.batches[k].step(self.chunk)
+            for k in active:
+                b = self.batches[k]
+                b.synchronize()
+                t, steps, status, done, used = b.poll()
+                for slot, r in list(resident[k].items()):
+                    if done[slot]:
+                        r.completed = time.perf_counter() - t0
+                        r.steps = int(steps[slot])
+                        r.sim_time = float(t[slot])
+                        r.status = int(status[slot])
+                        if self.keep:
+                            out = b.download(slot, r.spec.shape)
+                            if k == "dual":
+                                r.result, r.tangent = out
+                            else:
+                                r.result = out
+                        cell_updates += r.spec.n_cells * r.steps
+                        b.remove(slot)
+                        del resident[k][slot]
+        wall = time.perf_counter() - t0
+        lat = np.array([r.latency for r in requests])
+        return ServeReport(requests, wall, cell_updates, lat)
+
+
+def request_footprint(req: Request) -> int:
+    nr, nc = req.spec.shape
+    return footprint(req.kind, nr, nc, max(1, len(req.spec.sources)))

@@ -212,13 +212,19 @@ class Grid2D:
     device; ``set_state``/``get_state`` are the only host<->device copies.
     """
 
-    def __init__(self, n_rows: int, n_cols: int, dx: float, dy: float, dual: bool = False):
+    KINDS = {"f64": 0, "dual": 1, "f32": 2}
+
+    def __init__(self, n_rows: int, n_cols: int, dx: float, dy: float, dual: bool = False, kind: str = None):
+        kind = kind or ("dual" if dual else "f64")
+        if kind not in self.KINDS:
+            raise ValueError(f"kind must be one of {sorted(self.KINDS)}")
         h = C.c_void_p()
-        rc = lib().hf2d_create(int(n_rows), int(n_cols), float(dx), float(dy), 1 if dual else 0, C.byref(h))
+        rc = lib().hf2d_create(int(n_rows), int(n_cols), float(dx), float(dy), self.KINDS[kind], C.byref(h))
         _check(rc)
         self._h = h
         self.n_rows, self.n_cols, self.dx, self.dy = int(n_rows), int(n_cols), float(dx), float(dy)
-        self.dual = bool(dual)
+        self.kind = kind
+        self.dual = kind == "dual"
 
     @classmethod
     def for_mesh(cls, mesh: Mesh2D, dual: Optional[bool] = None) -> "Grid2D":
@@ -339,6 +345,12 @@ class Grid2D:
 
     def set_time(self, t: float, steps: int):
         _check(lib().hf2d_set_time(self._h, float(t), int(steps)), self._h)
+
+    def total_volume(self) -> float:
+        """sum(h) dx dy of the current state (FP64, compensated)."""
+        v = C.c_double()
+        _check(lib().hf2d_total_volume(self._h, C.byref(v)), self._h)
+        return v.value
 
     def set_kernel_timing(self, on: bool):
         _check(lib().hf2d_set_kernel_timing(self._h, 1 if on else 0), self._h)

@@ -35,11 +35,15 @@ enum hf2d_status {
   HF2D_DEGENERATE_DT = 3,          /* SimError::DegenerateDt */
   HF2D_STEP_BUDGET_EXHAUSTED = 4,  /* SimError::StepBudgetExhausted */
   HF2D_HIP_ERROR = 5,              /* device runtime failure */
-  HF2D_INVALID_ARGUMENT = 6        /* null handle / pointer, unsupported kind */
+  HF2D_INVALID_ARGUMENT = 6,       /* null handle / pointer, unsupported kind */
+  HF2D_CAPACITY = 7                /* batch: no free slot or the scenario does not fit in HBM */
 };
 
-/* Scalar of the grid: f64, or forward-mode Dual (autograd/src/dual.rs:12). */
-enum hf2d_scalar_kind { HF2D_F64 = 0, HF2D_DUAL = 1 };
+/* Scalar of the grid: f64, forward-mode Dual (autograd/src/dual.rs:12), or the
+ * FP32 precision form (no reference counterpart: FP32 state and flux
+ * arithmetic, bed as an FP64 datum per 64x64 block + FP32 deviation, FP64
+ * dt reduction and Kahan-compensated clock). */
+enum hf2d_scalar_kind { HF2D_F64 = 0, HF2D_DUAL = 1, HF2D_F32 = 2 };
 
 /* Boundary kinds (boundary.rs:80-106). Sides are ordered N, S, W, E
  * (Boundaries2D, boundary.rs:131-140). param is q for Discharge, h for
@@ -153,6 +157,11 @@ int hf2d_synchronize(hf2d_sim* sim);
 int hf2d_set_kernel_timing(hf2d_sim* sim, int32_t enabled);
 int hf2d_kernel_timing(hf2d_sim* sim, double stage_ms[2], int64_t* count);
 
+/* Total water volume sum(h) dx dy of the current state, summed in FP64 with
+ * compensation (the mass budget of the reference's tests, e.g.
+ * update.rs:1689-1717, computed on the device). */
+int hf2d_total_volume(hf2d_sim* sim, double* volume);
+
 /* Bandwidth reference: average duration of a plain device copy of n_bytes
  * (read n_bytes, write n_bytes) over `reps` launches. Also the calibration
  * kernel for hardware traffic counters. */
@@ -162,6 +171,53 @@ int hf2d_copy_probe(int64_t n_bytes, int32_t reps, double* ms_avg);
  * pow(x, 4.0/3.0) and pow(x, 4.0/3.0 - 1.0) as used by the step, written to
  * out[4*i + 0..3]. Lets a test check them bit-for-bit against the host. */
 int hf2d_math_probe(const double* x, const double* y, int64_t n, double* out);
+
+/* ------------------------------------------------------------------ batches
+ * Many independent scenarios (grids of different sizes, each with its own
+ * clock, dt, boundaries and forcing) resident in one HBM arena and advanced
+ * together: one persistent launch per RK stage over all their tiles, the
+ * step captured in a hipGraph. No counterpart in the reference, which runs
+ * one Simulation at a time; each scenario's arithmetic is that of
+ * hf2d_run_until (sim.rs:231-246) with the configured forcing. */
+typedef struct hf2d_batch hf2d_batch;
+
+int hf2d_batch_create(int32_t scalar_kind, int64_t capacity_bytes, int32_t max_slots, hf2d_batch** out);
+int hf2d_batch_destroy(hf2d_batch* batch);
+const char* hf2d_batch_last_error(const hf2d_batch* batch);
+
+/* HBM bytes a scenario occupies in a batch of the given kind (admission). */
+int hf2d_batch_footprint(int32_t scalar_kind, int64_t n_rows, int64_t n_cols, int64_t n_sources,
+                         int64_t* bytes);
+int hf2d_batch_free_bytes(const hf2d_batch* batch, int64_t* largest_block, int64_t* total);
+
+/* Admit a scenario that runs until its clock reaches t_end (SSP-RK2; the
+ * config's integrator must be HF2D_SSPRK2). HF2D_CAPACITY if no slot or
+ * memory is free. manning_tangent / tangent_aos: Dual kind only (nullable). */
+int hf2d_batch_add(hf2d_batch* batch, int64_t n_rows, int64_t n_cols, double dx, double dy,
+                   const double* bed, const double* manning, const double* manning_tangent,
+                   const double* state_aos, const double* tangent_aos, const int32_t bc_kinds[4],
+                   const double bc_params[4], const hf2d_config* config,
+                   const hf2d_point_source* sources, int64_t n_sources, double t_end,
+                   int32_t* slot);
+int hf2d_batch_remove(hf2d_batch* batch, int32_t slot);
+
+/* Enqueue n steps of every resident scenario (finished ones are no-ops);
+ * use_graph replays a captured 16-step graph. Returns without waiting. */
+int hf2d_batch_step(hf2d_batch* batch, int64_t n_steps, int32_t use_graph);
+int hf2d_batch_synchronize(hf2d_batch* batch);
+
+/* Per-slot clock and state of slots [0, n_slots): arrays may be null. */
+int hf2d_batch_poll(hf2d_batch* batch, int32_t n_slots, double* t, int64_t* steps, int32_t* status,
+                    int32_t* done, int32_t* used);
+int hf2d_batch_download(hf2d_batch* batch, int32_t slot, double* state_aos, double* tangent_aos);
+
+/* Tile-shape / threads-per-workgroup configurations and the autotuner that
+ * times each on the resident mix (states are restored afterwards). */
+int hf2d_batch_config_count(const hf2d_batch* batch, int32_t* count);
+int hf2d_batch_config_info(const hf2d_batch* batch, int32_t index, int32_t* tile_w, int32_t* tile_h,
+                           int32_t* threads);
+int hf2d_batch_set_config(hf2d_batch* batch, int32_t index);
+int hf2d_batch_autotune(hf2d_batch* batch, int32_t steps, int32_t* best, double* ms_per_step);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,190 @@
+"""GPU: batched scenarios and the serving loop.
+
+Every scenario in a batch runs the single-grid step's arithmetic with its own
+dt, so a scenario's result must not depend on what else is resident: the
+tests require bit-identical results alone vs in a mix, batch vs the
+single-grid driver, and (f64/Dual) batch vs the oracle.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import cases
+from oracle.pyoracle import OracleSim, set_pow_mode, set_threads
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(f"[gpu-case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(autouse=True)
+def _mode():
+    set_threads(16)
+    set_pow_mode(1)
+    yield
+    set_pow_mode(0)
+
+
+def _spec(sc, t_end, **kw):
+    from hydroflux_amd.batch import ScenarioSpec
+
+    return ScenarioSpec(bed=sc.bed, manning=sc.manning, dx=sc.dx, bc_kinds=sc.bc_kinds, rain_rate=sc.rain_rate,
+                        sources=sc.sources, t_end=t_end, max_dt=1.0, cfl=0.4, **kw)
+
+
+def _mix(n, seed, lo=40, hi=150):
+    from hydroflux_amd.scenarios import flood_scenario
+
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n):
+        r, c = int(rng.integers(lo, hi)), int(rng.integers(lo, hi))
+        out.append(flood_scenario(n=r, n_cols=c, dx=5.0, rain_mm_h=float(rng.uniform(10, 100)),
+                                  seed_dem=seed * 100 + 2 * k, seed_land=seed * 100 + 2 * k + 1))
+    return out
+
+
+def _run_batch(kind, specs, config=None, autotune=False):
+    from hydroflux_amd.batch import Batch
+
+    b = Batch(kind, capacity_bytes=1 << 30, max_slots=64)
+    if config is not None:
+        b.set_config(config)
+    slots = [b.add(s) for s in specs]
+    if autotune:
+        b.autotune(4)
+    for _ in range(400):
+        b.step(16)
+        b.synchronize()
+        done = b.poll()[3]
+        if all(done[s] for s in slots):
+            break
+    t, steps, status, done, used = b.poll()
+    outs = [b.download(s, sp.shape) for s, sp in zip(slots, specs)]
+    res = [(outs[k], float(t[s]), int(steps[s]), int(status[s])) for k, s in enumerate(slots)]
+    b.close()
+    return res
+
+
+def test_batch_f64_equals_single_grid_driver_and_oracle():
+    _say("batch f64 vs single grid vs oracle")
+    from hydroflux_amd import Grid2D
+
+    scs = _mix(3, 7)
+    specs = [_spec(sc, 30.0) for sc in scs]
+    res = _run_batch("f64", specs)
+    for sc, (out, t, steps, status) in zip(scs, res):
+        assert status == 0 and t == 30.0
+        for sim in (Grid2D(sc.n_rows, sc.n_cols, sc.dx, sc.dx), OracleSim(sc.n_rows, sc.n_cols, sc.dx, sc.dx)):
+            sim.set_mesh(sc.bed, sc.manning)
+            sim.set_state(sc.dry_state())
+            sim.set_boundaries(list(sc.bc_kinds))
+            sim.configure(cfl=0.4, max_dt=1.0, rain_rate=sc.rain_rate, sources=sc.sources)
+            sim.run_until(30.0)
+            assert sim.time() == (t, steps)
+            assert np.array_equal(sim.get_state(), out)
+
+
+def test_batch_f32_alone_equals_in_mix_and_across_tile_configs():
+    _say("batch f32 alone vs mix vs configs")
+    scs = _mix(6, 11)
+    specs = [_spec(sc, 40.0) for sc in scs]
+    alone = _run_batch("f32", specs[:1])[0]
+    mix = _run_batch("f32", specs)[0]
+    assert np.array_equal(alone[0], mix[0]) and alone[1:] == mix[1:]
+    # tile shapes whose sides divide the 64-cell datum block share the datum frame
+    for cfg in (1, 2, 3, 4):
+        other = _run_batch("f32", specs[:2], config=cfg)[0]
+        assert np.array_equal(other[0], alone[0]), cfg
+
+
+def test_batch_f32_matches_single_grid_f32():
+    _say("batch f32 vs single-grid f32")
+    from hydroflux_amd import Grid2D
+
+    sc = _mix(1, 5, 200, 300)[0]
+    out, t, steps, _ = _run_batch("f32", [_spec(sc, 25.0)])[0]
+    g = Grid2D(sc.n_rows, sc.n_cols, sc.dx, sc.dx, kind="f32")
+    g.set_mesh(sc.bed, sc.manning)
+    g.set_state(sc.dry_state())
+    g.set_boundaries(list(sc.bc_kinds))
+    g.configure(cfl=0.4, max_dt=1.0, rain_rate=sc.rain_rate, sources=sc.sources)
+    g.run_until(25.0)
+    assert g.time()[1] == steps
+    assert np.array_equal(g.get_state(), out)
+
+
+def test_batch_dual_matches_oracle():
+    _say("batch dual vs oracle")
+    scs = _mix(2, 13, 30, 60)
+    specs = [_spec(sc, 15.0, manning_tangent=(sc.land_class == 1).astype(float)) for sc in scs]
+    res = _run_batch("dual", specs)
+    for sc, sp, ((v, tg), t, steps, status) in zip(scs, specs, res):
+        o = OracleSim(sc.n_rows, sc.n_cols, sc.dx, sc.dx, dual=True)
+        o.set_mesh(sc.bed, sc.manning, manning_tangent=sp.manning_tangent)
+        o.set_state(sc.dry_state())
+        o.set_boundaries(list(sc.bc_kinds))
+        o.configure(cfl=0.4, max_dt=1.0, rain_rate=sc.rain_rate, sources=sc.sources)
+        o.run_until(15.0)
+        ov, ot = o.get_state()
+        assert o.time() == (t, steps)
+        assert np.array_equal(ov, v) and np.array_equal(ot, tg)
+
+
+def test_batch_autotune_restores_state():
+    _say("autotune restore")
+    scs = _mix(4, 17)
+    specs = [_spec(sc, 20.0) for sc in scs]
+    plain = _run_batch("f32", specs, config=0)
+    tuned = _run_batch("f32", specs, config=0, autotune=True)
+    # autotune may pick another tile shape; shapes dividing 64 agree bitwise, 128x2 to FP32 rounding
+    for (a, ta, sa, _), (b, tb, sb, _) in zip(plain, tuned):
+        assert ta == tb == 20.0
+        assert np.max(np.abs(a - b)) < 1e-3
+
+
+def test_batch_admission_by_footprint():
+    _say("admission")
+    from hydroflux_amd.batch import Batch, BatchFull, footprint
+
+    sc = _mix(1, 3, 100, 101)[0]
+    fp = footprint("f32", sc.n_rows, sc.n_cols, 1)
+    b = Batch("f32", capacity_bytes=2 * fp + 1000, max_slots=8)
+    s0 = b.add(_spec(sc, 5.0))
+    s1 = b.add(_spec(sc, 5.0))
+    with pytest.raises(BatchFull):
+        b.add(_spec(sc, 5.0))
+    b.remove(s0)
+    s2 = b.add(_spec(sc, 5.0))  # the freed block is reused
+    assert s2 == s0 and s1 != s0
+    b.close()
+
+
+def test_serving_results_equal_alone():
+    _say("serving poisson")
+    from hydroflux_amd.serving import Request, Server, poisson_arrivals
+
+    scs = _mix(10, 23, 40, 120)
+    arr = poisson_arrivals(len(scs), rate=40.0, seed=3)
+    reqs = []
+    for k, (sc, a) in enumerate(zip(scs, arr)):
+        dual = k % 4 == 3
+        sp = _spec(sc, float(10 + 2 * k), manning_tangent=((sc.land_class == 2).astype(float) if dual else None))
+        reqs.append(Request(k, float(a), sp, "dual" if dual else "f32"))
+    srv = Server({"f32": 1 << 28, "dual": 1 << 28}, max_slots=8, chunk_steps=8, tile_config={"f32": 0})
+    rep = srv.run(reqs)
+    srv.close()
+    s = rep.summary()
+    assert s["failed"] == 0 and s["requests"] == len(reqs)
+    assert np.all(np.isfinite(rep.latencies)) and s["latency_p99_s"] >= s["latency_p50_s"]
+    for r in reqs:
+        assert r.sim_time == r.spec.t_end
+        alone = _run_batch(r.kind, [r.spec], config=0)[0]
+        if r.kind == "dual":
+            assert np.array_equal(alone[0][0], r.result) and np.array_equal(alone[0][1], r.tangent)
+        else:
+            assert np.array_equal(alone[0], r.result)
