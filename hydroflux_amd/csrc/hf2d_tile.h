@@ -25,6 +25,8 @@
 #pragma once
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "hf_scalar.h"
 
 namespace hf {
@@ -377,6 +379,35 @@ template <class T, int TW, int TH> struct Stage {
   using S = TileSmem<T, TW, TH>;
   using R = typename ScalarOf<T>::type;
 
+  // FP32 precision form: depths are never recovered from an absolute water
+  // surface. eta = h + z with z of metres keeps only ~1e-7 m of h in FP32,
+  // which is a large fraction of a rain film; so face depths are formed as
+  // h + slope * d/2 + (z_cell - z_face) with the bed term from the bed
+  // difference, the face flux is evaluated in the face's own frame
+  // (z_face = 0), and the bed-slope source relative to the cell's bed. In
+  // exact arithmetic these are the reference's expressions; the f64 and Dual
+  // paths keep the reference's own operand order.
+  static constexpr bool kRel = std::is_same<T, float>::value;
+
+  // z_L - z_face and z_R - z_face of an interior face (update.rs:650-656):
+  // midpoint -> -+(z_R - z_L)/2; wet/dry face (z_face = max) -> min(., 0).
+  static HF_HD void interior_dz(bool l_dry, bool r_dry, T d, T* dzl, T* dzr) {
+    const T Z = cst<T>(0.0);
+    if (l_dry != r_dry) {
+      *dzl = rmin(d, Z);
+      *dzr = rmin(-d, Z);
+    } else {
+      *dzl = d * R(0.5);
+      *dzr = -(d * R(0.5));
+    }
+  }
+  // z_cell - z_face on a domain-edge face: (z_cell - z_ghost) / 2 with the
+  // ghost bed of boundary.rs:335-367.
+  static HF_HD T boundary_dz(int kind, T zc, T zn) {
+    if (kind == BC_TRANSMISSIVE || kind == BC_WALL) return cst<T>(0.0);
+    return (zn - zc) * R(0.5);
+  }
+
   // Face bed of x-face fj in local row li (update.rs:676-700).
   static HF_HD T zface_x(const S& s, const StageArgs<T>& a, int64_t gi0, int64_t gj0, int li, int fj) {
     const int64_t gjf = gj0 + fj;
@@ -468,7 +499,10 @@ template <class T, int TW, int TH> struct Stage {
         T de = Z, du = Z, dv = Z;
         if (gi >= 0 && gi < a.nr && gj >= 1 && gj < a.nc) {
           const int l = S::i3(li, fj - 1), r = S::i3(li, fj);
-          de = div(s.h[r] + s.z[r] - (s.h[l] + s.z[l]), dx);
+          if constexpr (kRel)
+            de = div((s.h[r] - s.h[l]) + (s.z[r] - s.z[l]), dx);
+          else
+            de = div(s.h[r] + s.z[r] - (s.h[l] + s.z[l]), dx);
           du = div(s.u[r] - s.u[l], dx);
           dv = div(s.v[r] - s.v[l], dx);
         }
@@ -482,7 +516,10 @@ template <class T, int TW, int TH> struct Stage {
         T de = Z, du = Z, dv = Z;
         if (gj >= 0 && gj < a.nc && gi >= 1 && gi < a.nr) {
           const int t = S::i3(fi - 1, lj), b = S::i3(fi, lj);
-          de = div(s.h[b] + s.z[b] - (s.h[t] + s.z[t]), dy);
+          if constexpr (kRel)
+            de = div((s.h[b] - s.h[t]) + (s.z[b] - s.z[t]), dy);
+          else
+            de = div(s.h[b] + s.z[b] - (s.h[t] + s.z[t]), dy);
           du = div(s.u[b] - s.u[t], dy);
           dv = div(s.v[b] - s.v[t], dy);
         }
@@ -598,6 +635,7 @@ template <class T, int TW, int TH> struct Stage {
         const int side = west ? SIDE_W : SIDE_E;
         ghost_state(a.bc_kind[side], a.bc_param[side], side, h, hu, hv, zc, zn, a.nc >= 2, a.dx,
                     a.man.ld(gi * a.nc + gc), &gh, &ghu, &ghv);
+        if (kRel) zf = Z;
         if (west)
           well_balanced_face(gh, ghu, ghv, h, hu, hv, zf, &m, &na, &nb, &tg);
         else
@@ -608,14 +646,23 @@ template <class T, int TW, int TH> struct Stage {
         T sle, slu, slv, sre, sru, srv;
         slope_x(s, a, li, fj - 1, gjf - 1, &sle, &slu, &slv);
         slope_x(s, a, li, fj, gjf, &sre, &sru, &srv);
-        T eta_m = s.h[l] + s.z[l] + sle * half_dx;
         T u_m = s.u[l] + slu * half_dx;
         T v_m = s.v[l] + slv * half_dx;
-        T eta_p = s.h[r] + s.z[r] - sre * half_dx;
         T u_p = s.u[r] - sru * half_dx;
         T v_p = s.v[r] - srv * half_dx;
-        T h_m = rmax(eta_m - zf, Z);
-        T h_p = rmax(eta_p - zf, Z);
+        T h_m, h_p;
+        if constexpr (kRel) {
+          T dzl, dzr;
+          interior_dz(s.dry[S::i4(li, fj - 1)] != 0, s.dry[S::i4(li, fj)] != 0, s.z[l] - s.z[r], &dzl, &dzr);
+          h_m = rmax(s.h[l] + sle * half_dx + dzl, Z);
+          h_p = rmax(s.h[r] - sre * half_dx + dzr, Z);
+          zf = Z;
+        } else {
+          T eta_m = s.h[l] + s.z[l] + sle * half_dx;
+          T eta_p = s.h[r] + s.z[r] - sre * half_dx;
+          h_m = rmax(eta_m - zf, Z);
+          h_p = rmax(eta_p - zf, Z);
+        }
         well_balanced_face(h_m, h_m * u_m, h_m * v_m, h_p, h_p * u_p, h_p * v_p, zf, &m, &na, &nb, &tg);
       }
     }
@@ -646,6 +693,7 @@ template <class T, int TW, int TH> struct Stage {
         ghost_state(a.bc_kind[side], a.bc_param[side], side, h, hu, hv, zc, zn, a.nr >= 2, a.dy,
                     a.man.ld(gc * a.nc + gj), &gh, &ghu, &ghv);
         // y-face: normal momentum is hv, tangential hu (riemann.rs:97-107)
+        if (kRel) zf = Z;
         if (north)
           well_balanced_face(gh, ghv, ghu, h, hv, hu, zf, &m, &na, &nb, &tg);
         else
@@ -656,14 +704,23 @@ template <class T, int TW, int TH> struct Stage {
         T ste, stu, stv, sbe, sbu, sbv;
         slope_y(s, a, fi - 1, lj, gif - 1, &ste, &stu, &stv);
         slope_y(s, a, fi, lj, gif, &sbe, &sbu, &sbv);
-        T eta_m = s.h[t] + s.z[t] + ste * half_dy;
         T u_m = s.u[t] + stu * half_dy;
         T v_m = s.v[t] + stv * half_dy;
-        T eta_p = s.h[b] + s.z[b] - sbe * half_dy;
         T u_p = s.u[b] - sbu * half_dy;
         T v_p = s.v[b] - sbv * half_dy;
-        T h_m = rmax(eta_m - zf, Z);
-        T h_p = rmax(eta_p - zf, Z);
+        T h_m, h_p;
+        if constexpr (kRel) {
+          T dzt, dzb;
+          interior_dz(s.dry[S::i4(fi - 1, lj)] != 0, s.dry[S::i4(fi, lj)] != 0, s.z[t] - s.z[b], &dzt, &dzb);
+          h_m = rmax(s.h[t] + ste * half_dy + dzt, Z);
+          h_p = rmax(s.h[b] - sbe * half_dy + dzb, Z);
+          zf = Z;
+        } else {
+          T eta_m = s.h[t] + s.z[t] + ste * half_dy;
+          T eta_p = s.h[b] + s.z[b] - sbe * half_dy;
+          h_m = rmax(eta_m - zf, Z);
+          h_p = rmax(eta_p - zf, Z);
+        }
         well_balanced_face(h_m, h_m * v_m, h_m * u_m, h_p, h_p * v_p, h_p * u_p, zf, &m, &na, &nb, &tg);
       }
     }
@@ -870,24 +927,61 @@ template <class T, int TW, int TH> struct Stage {
         T dhu = (ra - lb) * dt_dx + (bt - tt) * dt_dy;
         T dhv = (rt - lt) * dt_dx + (ba - tb) * dt_dy;
 
-        T eta_cell = h_old + s.z[c3];
         T sex = slope_x_eta(s, a, li, lj, gj);
-        T eta_left = (gj > 0) ? eta_cell - sex * R(0.5 * a.dx) : eta_cell;
-        T eta_right = (gj + 1 < a.nc) ? eta_cell + sex * R(0.5 * a.dx) : eta_cell;
-        T hfl = rmax(eta_left - zface_x(s, a, gi0, gj0, li, lj), Z);
-        T hfr = rmax(eta_right - zface_x(s, a, gi0, gj0, li, lj + 1), Z);
-        T zel = eta_left - hfl;
-        T zer = eta_right - hfr;
-        T s_hu = div((hfl + hfr) * (zel - zer) * R(0.5 * GRAVITY), R(a.dx));
-
         T sey = slope_y_eta(s, a, li, lj, gi);
-        T eta_top = (gi > 0) ? eta_cell - sey * R(0.5 * a.dy) : eta_cell;
-        T eta_bot = (gi + 1 < a.nr) ? eta_cell + sey * R(0.5 * a.dy) : eta_cell;
-        T hft = rmax(eta_top - zface_y(s, a, gi0, gj0, li, lj), Z);
-        T hfb = rmax(eta_bot - zface_y(s, a, gi0, gj0, li + 1, lj), Z);
-        T zet = eta_top - hft;
-        T zeb = eta_bot - hfb;
-        T s_hv = div((hft + hfb) * (zet - zeb) * R(0.5 * GRAVITY), R(a.dy));
+        T s_hu, s_hv;
+        if constexpr (kRel) {
+          // Same source in the cell's own bed frame: e = eta - z_cell and
+          // dz = z_cell - z_face, so h_face = max(e + dz, 0) and
+          // z_eff - z_cell = e - h_face.
+          const T zc = s.z[c3];
+          T dzl, dzr, dzt, dzb, tmp;
+          if (gj > 0)
+            interior_dz(s.dry[S::i4(li, lj - 1)] != 0, s.dry[S::i4(li, lj)] != 0, s.z[S::i3(li, lj - 1)] - zc, &tmp,
+                        &dzl);
+          else
+            dzl = boundary_dz(a.bc_kind[SIDE_W], zc, a.nc >= 2 ? s.z[S::i3(li, lj + 1)] : zc);
+          if (gj + 1 < a.nc)
+            interior_dz(s.dry[S::i4(li, lj)] != 0, s.dry[S::i4(li, lj + 1)] != 0, zc - s.z[S::i3(li, lj + 1)], &dzr,
+                        &tmp);
+          else
+            dzr = boundary_dz(a.bc_kind[SIDE_E], zc, a.nc >= 2 ? s.z[S::i3(li, lj - 1)] : zc);
+          if (gi > 0)
+            interior_dz(s.dry[S::i4(li - 1, lj)] != 0, s.dry[S::i4(li, lj)] != 0, s.z[S::i3(li - 1, lj)] - zc, &tmp,
+                        &dzt);
+          else
+            dzt = boundary_dz(a.bc_kind[SIDE_N], zc, a.nr >= 2 ? s.z[S::i3(li + 1, lj)] : zc);
+          if (gi + 1 < a.nr)
+            interior_dz(s.dry[S::i4(li, lj)] != 0, s.dry[S::i4(li + 1, lj)] != 0, zc - s.z[S::i3(li + 1, lj)], &dzb,
+                        &tmp);
+          else
+            dzb = boundary_dz(a.bc_kind[SIDE_S], zc, a.nr >= 2 ? s.z[S::i3(li - 1, lj)] : zc);
+          T el = (gj > 0) ? h_old - sex * R(0.5 * a.dx) : h_old;
+          T er = (gj + 1 < a.nc) ? h_old + sex * R(0.5 * a.dx) : h_old;
+          T hfl = rmax(el + dzl, Z), hfr = rmax(er + dzr, Z);
+          s_hu = div((hfl + hfr) * ((el - hfl) - (er - hfr)) * R(0.5 * GRAVITY), R(a.dx));
+          T et = (gi > 0) ? h_old - sey * R(0.5 * a.dy) : h_old;
+          T eb = (gi + 1 < a.nr) ? h_old + sey * R(0.5 * a.dy) : h_old;
+          T hft = rmax(et + dzt, Z), hfb = rmax(eb + dzb, Z);
+          s_hv = div((hft + hfb) * ((et - hft) - (eb - hfb)) * R(0.5 * GRAVITY), R(a.dy));
+        } else {
+          T eta_cell = h_old + s.z[c3];
+          T eta_left = (gj > 0) ? eta_cell - sex * R(0.5 * a.dx) : eta_cell;
+          T eta_right = (gj + 1 < a.nc) ? eta_cell + sex * R(0.5 * a.dx) : eta_cell;
+          T hfl = rmax(eta_left - zface_x(s, a, gi0, gj0, li, lj), Z);
+          T hfr = rmax(eta_right - zface_x(s, a, gi0, gj0, li, lj + 1), Z);
+          T zel = eta_left - hfl;
+          T zer = eta_right - hfr;
+          s_hu = div((hfl + hfr) * (zel - zer) * R(0.5 * GRAVITY), R(a.dx));
+
+          T eta_top = (gi > 0) ? eta_cell - sey * R(0.5 * a.dy) : eta_cell;
+          T eta_bot = (gi + 1 < a.nr) ? eta_cell + sey * R(0.5 * a.dy) : eta_cell;
+          T hft = rmax(eta_top - zface_y(s, a, gi0, gj0, li, lj), Z);
+          T hfb = rmax(eta_bot - zface_y(s, a, gi0, gj0, li + 1, lj), Z);
+          T zet = eta_top - hft;
+          T zeb = eta_bot - hfb;
+          s_hv = div((hft + hfb) * (zet - zeb) * R(0.5 * GRAVITY), R(a.dy));
+        }
 
         if (val(new_h) <= R(H_VEL)) {
           nh = rmax(new_h, Z);

@@ -60,7 +60,7 @@ HF_HD dual operator/(dual a, double b) { return {a.v / b, a.d / b}; }
 HF_HD double div(double a, double b) { return a / b; }
 HF_HD dual div(dual a, dual b) { return a / b; }
 HF_HD dual div(dual a, double b) { return a / b; }
-#ifdef __HIP_DEVICE_COMPILE__
+#if defined(__HIP_DEVICE_COMPILE__) && !defined(HF_F32_IEEE)
 HF_HD float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 HF_HD float fsqrt_(float a) { return __builtin_amdgcn_sqrtf(a); }
 #else
