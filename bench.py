@@ -1,48 +1,79 @@
-"""Benchmark: SSP-RK2 flood step on one MI355X, cell-updates/s and HBM roofline.
+"""Benchmark: SSP-RK2 flood step on one MI355X — cell-updates/s and HBM
+roofline for FP64 and the FP32 precision form, the 256-scenario batch, and
+the continuous-batching serving loop's p50/p99 latency.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--size 4096] [--no-cpu]
+                    [--quick] [--acc-steps 2000] [--serve-requests 64]
 
-Workload (config.workload): the rain-on-grid flood construction — seeded fBm
-DEM (H 0.8, 50 m relief) + 0.01 slope + 8 m carved valley, +2000 m bed
-offset, Manning n from 4 land-cover classes in Voronoi patches, initially
-dry, 50 mm/h rain, 100 m^3/s point inflow, walls N/S/W and open E — at
+Headline (`value`): the rain-on-grid flood construction — seeded fBm DEM
+(H 0.8, 50 m relief) + 0.01 slope + 8 m carved valley, +2000 m bed offset,
+Manning n from 4 land-cover classes in Voronoi patches, initially dry,
+50 mm/h rain, 100 m^3/s point inflow, walls N/S/W and open E — at
 4096 x 4096 cells, dx 5 m, in FP64: the largest single-GPU plain-FP64
-configuration, and a footprint (1.1 GB) well beyond the 256 MB Infinity
-Cache, so HBM bandwidth is what is measured. Each step is one SSP-RK2 step
-with Manning friction, rain and the point source, dt from the device-side
-CFL reduction (no host round trip), after W warm-up steps that wet the
-domain. Inputs are resident in HBM before the timed region.
+configuration, with a footprint (1.1 GB) well beyond the 256 MB Infinity
+Cache, so HBM bandwidth is what the roofline fraction is taken against.
+Each step is one SSP-RK2 step with Manning friction, rain and the point
+source, dt from the device-side CFL reduction (no host round trip), after W
+warm-up steps that wet the domain. Inputs are resident in HBM before the
+timed region.
+
+Beside it in the same JSON line (unless --quick):
+  precision_form  the same 4096^2 workload in the FP32 precision form
+                  (throughput, roofline), and its depth accuracy against the
+                  FP64 run at the same simulated time after --acc-steps FP64
+                  steps, next to the FP64 run's own sensitivity to a 1e-9 m
+                  bed perturbation (the conditioning floor of the workload);
+  c1_1024         the 1024^2 workload (no offset) in FP64 and Dual;
+  batch           256 scenarios of mixed size (sides log-uniform in
+                  [128, 2048], aspect U(1, 4)), each with its own DEM, land
+                  cover, rain and dt: FP32 form and FP64, autotuned tile
+                  shape, against serial per-scenario FP64 launches;
+  serving         Poisson arrivals of requests drawn from that mix, 25 % with
+                  Dual tangents on one land-cover class's Manning n, at a
+                  sweep of arrival rates: p50/p99 latency-to-completion and
+                  aggregate cell-updates/s;
+  cpu_baseline    the C++ restatement of the reference step, 1 thread, on a
+                  bounded 1024^2 sample.
 
 One JSON line on stdout; progress on stderr.
 """
 from __future__ import annotations
 
 import argparse
+import ctypes as C
 import json
 import os
 import sys
 import time
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "cell-updates/s 2D SSP-RK2 (FP64 vs FP32) + %HBM peak; batched p50/p99 latency"
 HBM_PEAK_GBS = 8000.0
-# Algorithmic HBM bytes per cell for one launch, FP64 (8 B per value):
-#   first stage : read h, hu, hv, z; write U(1)                    = 7 x 8 = 56
-#   final stage : read U(1), z, U^n, n; write U^{n+1}             = 11 x 8 = 88
-ALG_BYTES = {"stage1": 56.0, "final": 88.0, "step": 144.0}
+# Algorithmic HBM bytes per cell for one launch (each field read or written once):
+#   FP64 first stage : read h, hu, hv, z; write U(1)                 = 7 x 8  = 56
+#   FP64 final stage : read U(1), z, U^n, n; write U^{n+1}          = 11 x 8 = 88
+#   FP32 first stage : read h, hu, hv, z deviation; write U(1)       = 7 x 4  = 28
+#   FP32 final stage : read U(1), z dev., U^n, n; write U^{n+1}     = 11 x 4 = 44
+ALG_BYTES = {
+    "f64": {"stage1": 56.0, "final": 88.0, "step": 144.0},
+    "f32": {"stage1": 28.0, "final": 44.0, "step": 72.0},
+}
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def build_grid(sc):
+def build_grid(sc, kind="f64", bed=None):
     from hydroflux_amd import Grid2D
 
-    g = Grid2D(sc.n_rows, sc.n_cols, sc.dx, sc.dx)
-    g.set_mesh(sc.bed, sc.manning)
+    g = Grid2D(sc.n_rows, sc.n_cols, sc.dx, sc.dx, kind=kind)
+    g.set_mesh(sc.bed if bed is None else bed, sc.manning,
+               manning_tangent=(sc.manning if kind == "dual" else None))
     g.set_state(sc.dry_state())
     g.set_boundaries(list(sc.bc_kinds))
     g.configure(cfl=sc.cfl, max_dt=sc.max_dt, friction=True, friction_dry_tol=1e-6, rain_rate=sc.rain_rate,
@@ -51,6 +82,9 @@ def build_grid(sc):
 
 
 def time_steps(g, warmup, steps):
+    """Wall time of `steps` enqueued steps after `warmup` steps, and the
+    per-launch averages of the two stage kernels from HIP events recorded
+    on the grid's own stream."""
     g.run_steps(warmup)
     g.synchronize()
     g.set_kernel_timing(True)
@@ -60,7 +94,26 @@ def time_steps(g, warmup, steps):
     t1 = time.perf_counter()
     s1, s2, cnt = g.kernel_timing()
     g.set_kernel_timing(False)
-    return (t1 - t0), s1, s2, cnt
+    return (t1 - t0), s1 / max(cnt, 1), s2 / max(cnt, 1)
+
+
+def roofline(kind, cells, t_first_ms, t_final_ms, traffic, kernel):
+    ab = ALG_BYTES[kind]
+    achieved = ab["final"] * cells / (t_final_ms / 1e3) / 1e9
+    return {
+        "bound": "hbm",
+        "kernel": kernel,
+        "achieved": achieved,
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": achieved / HBM_PEAK_GBS,
+        "traffic": traffic,
+        "alg_bytes_per_cell": ab["final"],
+        "cells_per_launch": cells,
+        "avg_launch_ms": t_final_ms,
+        "first_stage": {"alg_bytes_per_cell": ab["stage1"], "avg_launch_ms": t_first_ms,
+                        "achieved": ab["stage1"] * cells / (t_first_ms / 1e3) / 1e9},
+    }
 
 
 def cpu_baseline(n=1024, warmup=3, steps=20):
@@ -92,6 +145,7 @@ def cpu_baseline(n=1024, warmup=3, steps=20):
                   f"{warmup} warm-up + {steps} timed SSP-RK2+friction+rain+inflow steps ({dt:.1f} s)",
         "ns_per_cell_step": dt / (sc.n_cells * steps) * 1e9,
         "host_cpu": _cpu_model(),
+        "host_cores": os.cpu_count(),
     }
 
 
@@ -105,16 +159,164 @@ def _cpu_model():
     return None
 
 
-def _traffic():
-    """HBM bytes per launch of the final stage kernel from the committed
-    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (copy-kernel calibrated)."""
+def _traffic(key):
+    """HBM bytes per launch of a stage kernel from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (copy-kernel calibrated)."""
     p = os.path.join(ROOT, "profiles", "pmc_traffic.json")
     try:
         with open(p) as f:
             d = json.load(f)
-        return d.get("final_stage_bytes_per_launch")
+        return d.get(key)
     except (OSError, ValueError):
         return None
+
+
+def precision_form(sc, g64, args):
+    """FP32 form on the headline workload: throughput, roofline, and depth
+    accuracy against the FP64 grid `g64` (already advanced warmup + steps)."""
+    from hydroflux_amd.metrics import depth_metrics
+
+    cells = sc.n_cells
+    log("[bench] FP32 precision form: timing")
+    g32 = build_grid(sc, "f32")
+    wall, t1, t2 = time_steps(g32, args.warmup, args.steps)
+    out = {
+        "value": cells * args.steps / wall,
+        "ms_per_step": wall / args.steps * 1e3,
+        "dtype": "f32 state/flux, f64 bed datum per 64x64 block, f64 dt/time/mass",
+        "roofline": roofline("f32", cells, t1, t2, _traffic("f32_final_stage_bytes_per_launch"),
+                             "stage_kernel<float, final>"),
+    }
+    if args.acc_steps > 0:
+        done = args.warmup + args.steps
+        log(f"[bench] accuracy: FP64 to {args.acc_steps} steps, FP32 and perturbed FP64 to the same time")
+        g64.run_steps(max(0, args.acc_steps - done))
+        t_ref, n_ref = g64.time()
+        s64 = g64.get_state()[..., 0]
+        v64 = g64.total_volume()
+        g32.run_until(t_ref)
+        m32 = depth_metrics(g32.get_state()[..., 0], s64, sc.dx, sc.dx, vol_test=g32.total_volume(), vol_ref=v64)
+        n32 = g32.time()[1]
+        g32.close()
+        rng = np.random.default_rng(12345)
+        gp = build_grid(sc, "f64", bed=sc.bed + 1e-9 * rng.standard_normal(sc.bed.shape))
+        gp.run_until(t_ref)
+        mp = depth_metrics(gp.get_state()[..., 0], s64, sc.dx, sc.dx, vol_test=gp.total_volume(), vol_ref=v64)
+        gp.close()
+        out["accuracy"] = {
+            "t_simulated_s": t_ref,
+            "fp64_steps": n_ref,
+            "fp32_steps": n32,
+            "fp32_vs_fp64": m32,
+            "fp64_bed_perturbed_1e-9m_vs_fp64": mp,
+        }
+    else:
+        g32.close()
+    return out
+
+
+def c1_1024(args):
+    """The 1024^2 workload (no bed offset) in FP64 and Dual (tangent seeded
+    on a global Manning multiplier: n-tangent = n)."""
+    from hydroflux_amd.scenarios import flood_scenario
+
+    sc = flood_scenario(n=1024, dx=5.0)
+    out = {}
+    for kind in ("f64", "dual"):
+        g = build_grid(sc, kind)
+        w, _, _ = time_steps(g, 10, 50)
+        out[kind] = sc.n_cells * 50 / w
+        g.close()
+    out["dual_over_f64_time"] = out["f64"] / out["dual"]
+    return out
+
+
+def batch_bench(args):
+    from hydroflux_amd import Grid2D
+    from hydroflux_amd.batch import Batch, ScenarioSpec
+    from hydroflux_amd.scenarios import scenario_mix
+
+    t0 = time.time()
+    mix = scenario_mix(256)
+    cells = sum(s.n_cells for s in mix)
+    log(f"[bench] batch: 256 scenarios, {cells / 1e6:.1f} M cells (generated in {time.time() - t0:.1f} s)")
+
+    def specs():
+        return [ScenarioSpec(bed=s.bed, manning=s.manning, dx=s.dx, bc_kinds=s.bc_kinds, rain_rate=s.rain_rate,
+                             sources=s.sources, t_end=1e9, max_dt=s.max_dt, cfl=s.cfl) for s in mix]
+
+    out = {"scenarios": len(mix), "cells": cells}
+    for kind in ("f32", "f64"):
+        b = Batch(kind, capacity_bytes=24 << 30, max_slots=512)
+        for sp in specs():
+            b.add(sp)
+        b.step(8)
+        b.synchronize()
+        best, ms = b.autotune(6)
+        b.step(16)
+        b.synchronize()
+        n = 64
+        t0 = time.perf_counter()
+        b.step(n)
+        b.synchronize()
+        w = time.perf_counter() - t0
+        cfgs = b.configs()
+        out[kind] = {"value": cells * n / w, "ms_per_step": w / n * 1e3, "tile": cfgs[best], "tile_index": int(best),
+                     "autotune_ms_per_step": {f"{c[0]}x{c[1]}x{c[2]}": float(m) for c, m in zip(cfgs, ms)}}
+        b.close()
+        log(f"[bench] batch {kind}: {cells * n / w / 1e9:.2f} G cell-updates/s, tile {cfgs[best]}")
+    # serial per-scenario plain FP64 launches (one grid and driver call per scenario)
+    tot_c, tot_w = 0, 0.0
+    for s in mix:
+        g = Grid2D(s.n_rows, s.n_cols, s.dx, s.dx)
+        g.set_mesh(s.bed, s.manning)
+        g.set_state(s.dry_state())
+        g.set_boundaries(list(s.bc_kinds))
+        g.configure(cfl=s.cfl, max_dt=s.max_dt, rain_rate=s.rain_rate, sources=s.sources)
+        g.run_steps(8)
+        g.synchronize()
+        t0 = time.perf_counter()
+        g.run_steps(16)
+        tot_w += time.perf_counter() - t0
+        tot_c += s.n_cells * 16
+        g.close()
+    out["serial_f64"] = tot_c / tot_w
+    out["f32_over_serial_f64"] = out["f32"]["value"] / out["serial_f64"]
+    out["f64_over_serial_f64"] = out["f64"]["value"] / out["serial_f64"]
+    log(f"[bench] serial f64: {out['serial_f64'] / 1e9:.2f} G cell-updates/s")
+    return out, mix
+
+
+def serving_bench(args, mix, f32_tile):
+    from hydroflux_amd.batch import ScenarioSpec
+    from hydroflux_amd.serving import Request, Server, poisson_arrivals
+
+    rng = np.random.default_rng(0x5E4F)
+    idx = rng.choice(len(mix), size=min(args.serve_requests, len(mix)), replace=False)
+    lo, hi = args.serve_duration
+    base = []
+    for k, i in enumerate(idx):
+        s = mix[i]
+        dual = k % 4 == 3
+        spec = ScenarioSpec(bed=s.bed, manning=s.manning, dx=s.dx, bc_kinds=s.bc_kinds, rain_rate=s.rain_rate,
+                            sources=s.sources, t_end=float(rng.uniform(lo, hi)), max_dt=s.max_dt, cfl=s.cfl,
+                            manning_tangent=((s.land_class == 2).astype(np.float64) if dual else None))
+        base.append((spec, "dual" if dual else "f32"))
+    out = {"requests": len(base), "dual_fraction": sum(k == "dual" for _, k in base) / len(base),
+           "duration_s": [lo, hi], "rates": []}
+    for rate in args.serve_rates:
+        arr = poisson_arrivals(len(base), rate=rate, seed=7)
+        reqs = [Request(k, float(a), sp, kind) for k, ((sp, kind), a) in enumerate(zip(base, arr))]
+        srv = Server({"f32": 16 << 30, "dual": 8 << 30}, max_slots=256, chunk_steps=32, keep_results=False,
+                     tile_config={"f32": f32_tile})
+        rep = srv.run(reqs)
+        srv.close()
+        s = rep.summary()
+        s["rate_req_per_s"] = None if not np.isfinite(rate) else rate
+        out["rates"].append(s)
+        log(f"[bench] serving rate {rate}: p50 {s['latency_p50_s']:.2f} s, p99 {s['latency_p99_s']:.2f} s, "
+            f"{s['cell_updates_per_s'] / 1e9:.2f} G cell-updates/s")
+    return out
 
 
 def main():
@@ -124,11 +326,18 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--size", type=int, default=4096)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--secondary", action="store_true", help="also time 1024^2 f64 and Dual")
+    ap.add_argument("--quick", action="store_true", help="headline FP64 line only (and cpu_baseline)")
+    ap.add_argument("--sections", default="precision,c1,batch,serving",
+                    help="comma list of extra sections: precision, c1, batch, serving")
+    ap.add_argument("--acc-steps", type=int, default=2000, help="FP64 steps for the FP32 accuracy check (0: skip)")
+    ap.add_argument("--serve-requests", type=int, default=64)
+    ap.add_argument("--serve-duration", type=float, nargs=2, default=(30.0, 120.0))
+    ap.add_argument("--serve-rates", type=float, nargs="+", default=(4.0, 16.0, float("inf")))
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(a single grid would shard by row strips with halo exchange; see DESIGN.md)", file=sys.stderr)
+              "(a single grid would shard by row strips with halo exchange, scenarios by footprint)",
+              file=sys.stderr)
         sys.exit(2)
 
     from hydroflux_amd._lib import lib
@@ -138,17 +347,12 @@ def main():
     log(f"[bench] building {n}x{n} flood scenario")
     sc = flood_scenario(n=n, dx=5.0, offset=2000.0)
     g = build_grid(sc)
-    log(f"[bench] warmup {args.warmup} + timed {args.steps} steps")
-    wall, s1_ms, s2_ms, cnt = time_steps(g, args.warmup, args.steps)
+    log(f"[bench] FP64: warmup {args.warmup} + timed {args.steps} steps")
+    wall, t_first, t_final = time_steps(g, args.warmup, args.steps)
     cells = sc.n_cells
     value = cells * args.steps / wall
-    t_final = s2_ms / max(cnt, 1) / 1e3
-    t_first = s1_ms / max(cnt, 1) / 1e3
-    achieved = ALG_BYTES["final"] * cells / t_final / 1e9
     log(f"[bench] {value / 1e9:.3f} G cell-updates/s, {wall / args.steps * 1e3:.3f} ms/step, "
-        f"stage1 {t_first * 1e3:.3f} ms, final {t_final * 1e3:.3f} ms")
-
-    import ctypes as C
+        f"stage1 {t_first:.3f} ms, final {t_final:.3f} ms")
 
     ms = C.c_double()
     copy_bytes = 1 << 30
@@ -156,6 +360,10 @@ def main():
     copy_gbs = 2 * copy_bytes / (ms.value / 1e3) / 1e9
     log(f"[bench] copy probe {copy_gbs:.0f} GB/s")
 
+    rf = roofline("f64", cells, t_first, t_final, _traffic("final_stage_bytes_per_launch"),
+                  "stage_kernel<double, final> (final SSP-RK2 stage + combine + forcing + CFL reduction)")
+    rf["step_hbm_frac"] = value * ALG_BYTES["f64"]["step"] / 1e9 / HBM_PEAK_GBS
+    rf["copy_probe_GBs"] = copy_gbs
     out = {
         "metric": METRIC,
         "value": value,
@@ -177,34 +385,21 @@ def main():
             "precision": "fp64 (bit-identical to the CPU restatement)",
             "parallelism": "single GPU",
         },
-        "roofline": {
-            "bound": "hbm",
-            "kernel": "stage_kernel<double, final> (final SSP-RK2 stage + combine + forcing + CFL reduction)",
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": _traffic(),
-            "alg_bytes_per_cell": ALG_BYTES["final"],
-            "avg_launch_ms": t_final * 1e3,
-            "first_stage": {"alg_bytes_per_cell": ALG_BYTES["stage1"], "avg_launch_ms": t_first * 1e3,
-                            "achieved": ALG_BYTES["stage1"] * cells / t_first / 1e9},
-            "step_hbm_frac": value * ALG_BYTES["step"] / 1e9 / HBM_PEAK_GBS,
-            "copy_probe_GBs": copy_gbs,
-        },
+        "roofline": rf,
     }
-    if args.secondary:
-        for dual in (False, True):
-            sc1 = flood_scenario(n=1024, dx=5.0)
-            from hydroflux_amd import Grid2D
-
-            g1 = Grid2D(1024, 1024, 5.0, 5.0, dual=dual)
-            g1.set_mesh(sc1.bed, sc1.manning, manning_tangent=(sc1.manning if dual else None))
-            g1.set_state(sc1.dry_state())
-            g1.set_boundaries(list(sc1.bc_kinds))
-            g1.configure(cfl=0.4, max_dt=1.0, rain_rate=sc1.rain_rate, sources=sc1.sources)
-            w1, _, _, _ = time_steps(g1, 10, 50)
-            out.setdefault("secondary", {})["dual_1024" if dual else "f64_1024"] = 1024 * 1024 * 50 / w1
+    sections = set() if args.quick else set(x for x in args.sections.split(",") if x)
+    if "precision" in sections:
+        out["precision_form"] = precision_form(sc, g, args)
+        out["precision_form"]["over_fp64"] = out["precision_form"]["value"] / value
+    g.close()
+    if "c1" in sections:
+        log("[bench] 1024^2 FP64 and Dual")
+        out["c1_1024"] = c1_1024(args)
+    if "batch" in sections or "serving" in sections:
+        out["batch"], mix = batch_bench(args)
+        if "serving" in sections:
+            out["serving"] = serving_bench(args, mix, out["batch"]["f32"]["tile_index"])
+            out["serving"]["serial_f64"] = out["batch"]["serial_f64"]
     if not args.no_cpu:
         log("[bench] CPU baseline (oracle, 1 thread, 1024^2 sample)")
         out["cpu_baseline"] = cpu_baseline()
