@@ -231,6 +231,35 @@ def c1_1024(args):
     return out
 
 
+def dry_domain(args):
+    """The compacted active-tile list (the reference's cell-mask early skip
+    at tile granularity): the 4096^2 flood construction without rain — only
+    the 100 m^3/s inflow wets the valley — timed with the list on and off
+    after a warm-up that spreads the inflow. Same results either way;
+    cell-updates count every cell of the grid."""
+    from hydroflux_amd.scenarios import flood_scenario
+
+    sc = flood_scenario(n=args.size, dx=5.0, offset=2000.0, rain_mm_h=0.0)
+    out = {"workload": f"flood DEM {args.size}x{args.size}, +2000 m, no rain, 100 m3/s inflow, "
+                       f"after {args.dry_warmup} steps"}
+    for skip in (True, False):
+        g = build_grid(sc)
+        g.configure(cfl=sc.cfl, max_dt=sc.max_dt, friction=True, friction_dry_tol=1e-6, rain_rate=0.0,
+                    sources=sc.sources, active_tiles=skip)
+        wall, t1, t2 = time_steps(g, args.dry_warmup, args.steps)
+        h = g.get_state()[..., 0]
+        key = "skip" if skip else "no_skip"
+        out[key] = {"value": sc.n_cells * args.steps / wall, "ms_per_step": wall / args.steps * 1e3,
+                    "first_stage_ms": t1, "final_stage_ms": t2}
+        out["wet_cells"] = int(np.count_nonzero(h > 1e-6))
+        out["wet_fraction"] = out["wet_cells"] / h.size
+        g.close()
+    out["speedup"] = out["skip"]["value"] / out["no_skip"]["value"]
+    log(f"[bench] dry domain: list {out['skip']['value'] / 1e9:.2f} vs every tile "
+        f"{out['no_skip']['value'] / 1e9:.2f} G cell-updates/s ({out['wet_cells']} wet cells)")
+    return out
+
+
 def batch_bench(args):
     from hydroflux_amd import Grid2D
     from hydroflux_amd.batch import Batch, ScenarioSpec
@@ -327,8 +356,9 @@ def main():
     ap.add_argument("--size", type=int, default=4096)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="headline FP64 line only (and cpu_baseline)")
-    ap.add_argument("--sections", default="precision,c1,batch,serving",
-                    help="comma list of extra sections: precision, c1, batch, serving")
+    ap.add_argument("--sections", default="precision,c1,dry,batch,serving",
+                    help="comma list of extra sections: precision, c1, dry, batch, serving")
+    ap.add_argument("--dry-warmup", type=int, default=500)
     ap.add_argument("--acc-steps", type=int, default=2000, help="FP64 steps for the FP32 accuracy check (0: skip)")
     ap.add_argument("--serve-requests", type=int, default=64)
     ap.add_argument("--serve-duration", type=float, nargs=2, default=(30.0, 120.0))
@@ -395,6 +425,8 @@ def main():
     if "c1" in sections:
         log("[bench] 1024^2 FP64 and Dual")
         out["c1_1024"] = c1_1024(args)
+    if "dry" in sections:
+        out["dry_domain"] = dry_domain(args)
     if "batch" in sections or "serving" in sections:
         out["batch"], mix = batch_bench(args)
         if "serving" in sections:

@@ -27,7 +27,7 @@ class HfConfig(C.Structure):
         ("cfl_with_bcs", C.c_int32),
         ("max_dt", C.c_double),
         ("friction", C.c_int32),
-        ("reserved", C.c_int32),
+        ("options", C.c_int32),
         ("friction_dry_tol", C.c_double),
         ("max_steps", C.c_int64),
         ("rain_rate", C.c_double),

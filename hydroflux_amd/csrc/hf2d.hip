@@ -42,6 +42,13 @@ struct hf2d_sim {
   hf2d_config cfg{};
   bool ws_valid = false;  // ctrl->ws[parity] holds the current state's maxima
   bool ws_with_bcs = false;
+  // active-tile list (f64 and f32 kinds): per-tile wet flags of U^n, quiet
+  // run lengths, pinned tiles (edge, point source), scratch, the list and
+  // its length (tile_list[n_tiles]).
+  unsigned char *tile_wet = nullptr, *tile_run = nullptr, *tile_pin = nullptr, *tile_act = nullptr;
+  int* tile_list = nullptr;
+  int tiles_x = 0, tiles_y = 0;
+  bool list_valid = false;  // tile_wet / tile_run follow the current state
   std::string err;
   // kernel timing
   bool timing = false;
@@ -82,6 +89,15 @@ hf2d_config default_config() {
 
 // Arguments for a launch reading state buffer `in`, writing buffer `out`,
 // with U^n in buffer `un`.
+// The active-tile list is used when it cannot change a bit (hf2d_tile.h,
+// tile_skipped): f64/f32 state, SSP-RK2, no rain, no friction on dry
+// cells, not switched off.
+bool list_usable(const hf2d_sim* s) {
+  return s->tile_list != nullptr && s->cfg.integrator == HF2D_SSPRK2 && s->cfg.rain_rate == 0.0 &&
+         (!s->cfg.friction || s->cfg.friction_dry_tol >= hf::H_DRY) &&
+         (s->cfg.options & HF2D_OPT_NO_ACTIVE_TILES) == 0;
+}
+
 template <class T> StageArgs<T> make_args(hf2d_sim* s, int in, int out, int un) {
   StageArgs<T> a{};
   a.nr = s->nr;
@@ -224,6 +240,27 @@ template <class T> int host_dt_step(hf2d_sim* s, double dt, bool rk2) {
     s->cur = oth;
   }
   s->ws_valid = false;
+  s->list_valid = false;
+  return HF2D_OK;
+}
+
+// Rebuild the active-tile list for the next step (after seeding the wet
+// flags from the state if it was set by other means than the driver).
+template <class T> int enqueue_compact(hf2d_sim* s) {
+  const int64_t nt = (int64_t)s->tiles_x * s->tiles_y;
+  const unsigned g = (unsigned)((nt + 255) / 256 < 4096 ? (nt + 255) / 256 : 4096);
+  if (!s->list_valid) {
+    hipLaunchKernelGGL(fill_bytes_kernel, dim3(g), dim3(256), 0, s->stream, s->tile_wet, nt, (unsigned char)0);
+    hipLaunchKernelGGL(fill_bytes_kernel, dim3(g), dim3(256), 0, s->stream, s->tile_run, nt, (unsigned char)0);
+    hipLaunchKernelGGL((tile_wet_init_kernel<T>), dim3(tail_grid(s)), dim3(256), 0, s->stream,
+                       fld<T>(s->st[s->cur][0], s->tg[s->cur][0]), s->nr, s->nc, Shape<T>::TW, Shape<T>::TH,
+                       s->tiles_x, s->tile_wet);
+    HIPCHK(s, hipGetLastError());
+    s->list_valid = true;
+  }
+  hipLaunchKernelGGL(compact_kernel, dim3(1), dim3(1024), 0, s->stream, s->tile_wet, s->tile_pin, s->tile_run,
+                     s->tile_act, s->tiles_x, s->tiles_y, s->tile_list, s->tile_list + nt);
+  HIPCHK(s, hipGetLastError());
   return HF2D_OK;
 }
 
@@ -241,6 +278,16 @@ template <class T> int enqueue_step(hf2d_sim* s) {
   b.rain_rate = s->cfg.rain_rate;
   b.n_src = s->n_src;
   b.wave_speed = 1;
+  if (list_usable(s)) {
+    int rc = enqueue_compact<T>(s);
+    if (rc) return rc;
+    const int64_t nt = (int64_t)s->tiles_x * s->tiles_y;
+    a.tile_list = b.tile_list = s->tile_list;
+    a.n_work = b.n_work = s->tile_list + nt;
+    b.tile_wet = s->tile_wet;
+  } else {
+    s->list_valid = false;
+  }
   if (s->cfg.integrator == HF2D_SSPRK2) {
     int rc = launch_stage<T>(s, a, hf::MODE_STAGE, 0);
     if (rc) return rc;
@@ -388,6 +435,7 @@ template <class T> int forcing_only(hf2d_sim* s, double dt, int friction, double
   hipLaunchKernelGGL((tail_kernel<T>), dim3(tail_grid(s)), dim3(256), 0, s->stream, a, 0);
   HIPCHK(s, hipGetLastError());
   s->ws_valid = false;
+  s->list_valid = false;
   return HF2D_OK;
 }
 
@@ -416,6 +464,21 @@ int upload_sources(hf2d_sim* s, const hf2d_point_source* src, int64_t k) {
     HIPCHK(s, hipMemcpy(s->srcs, v.data(), (size_t)k * sizeof(PointSrc), hipMemcpyHostToDevice));
   }
   s->n_src = (int)k;
+  return HF2D_OK;
+}
+// Tiles the active-tile list never leaves out: near the domain edge, or
+// holding a point source of the configured driver.
+int upload_pins(hf2d_sim* s, const hf2d_point_source* src, int64_t k) {
+  if (!s->tile_pin) return HF2D_OK;
+  const int tw = s->kind == HF2D_F32 ? Shape<float>::TW : Shape<double>::TW;
+  const int th = s->kind == HF2D_F32 ? Shape<float>::TH : Shape<double>::TH;
+  std::vector<unsigned char> pin((size_t)s->tiles_x * s->tiles_y, 0);
+  for (int ty = 0; ty < s->tiles_y; ++ty)
+    for (int tx = 0; tx < s->tiles_x; ++tx)
+      pin[(size_t)ty * s->tiles_x + tx] = hf::tile_near_edge(s->nr, s->nc, tw, th, tx, ty) ? 1 : 0;
+  for (int64_t i = 0; i < k; ++i) pin[(size_t)(src[i].row / th) * s->tiles_x + (size_t)(src[i].col / tw)] = 1;
+  HIPCHK(s, hipStreamSynchronize(s->stream));
+  HIPCHK(s, hipMemcpy(s->tile_pin, pin.data(), pin.size(), hipMemcpyHostToDevice));
   return HF2D_OK;
 }
 }  // namespace
@@ -468,6 +531,18 @@ int hf2d_create(int64_t n_rows, int64_t n_cols, double dx, double dy, int32_t sc
     if (!zalloc((void**)&s->zt, bytes)) return cleanup(HF2D_HIP_ERROR);
     if (!zalloc(&s->mant, bytes)) return cleanup(HF2D_HIP_ERROR);
   }
+  if (scalar_kind != HF2D_DUAL) {
+    const int tw = scalar_kind == HF2D_F32 ? Shape<float>::TW : Shape<double>::TW;
+    const int th = scalar_kind == HF2D_F32 ? Shape<float>::TH : Shape<double>::TH;
+    s->tiles_x = (int)((n_cols + tw - 1) / tw);
+    s->tiles_y = (int)((n_rows + th - 1) / th);
+    const size_t nt = (size_t)s->tiles_x * s->tiles_y;
+    if (!zalloc((void**)&s->tile_wet, nt) || !zalloc((void**)&s->tile_run, nt) ||
+        !zalloc((void**)&s->tile_pin, nt) || !zalloc((void**)&s->tile_act, nt) ||
+        !zalloc((void**)&s->tile_list, (nt + 1) * sizeof(int)))
+      return cleanup(HF2D_HIP_ERROR);
+    if (upload_pins(s, nullptr, 0) != HF2D_OK) return cleanup(HF2D_HIP_ERROR);
+  }
   if (hipMalloc(&s->ctrl, sizeof(Ctrl)) != hipSuccess) return cleanup(HF2D_HIP_ERROR);
   Ctrl c{};
   c.t_target = 0.0;
@@ -486,7 +561,9 @@ int hf2d_destroy(hf2d_sim* s) {
       if (s->st[b][f]) (void)hipFree(s->st[b][f]);
       if (s->tg[b][f]) (void)hipFree(s->tg[b][f]);
     }
-  void* bufs[] = {s->z, s->zt, s->man, s->mant, s->datum, s->zdev, s->ctrl, s->srcs};
+  void* bufs[] = {s->z,        s->zt,       s->man,      s->mant,     s->datum,    s->zdev,
+                  s->ctrl,     s->srcs,     s->tile_wet, s->tile_run, s->tile_pin, s->tile_act,
+                  s->tile_list};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   for (hipEvent_t e : s->ev) (void)hipEventDestroy(e);
@@ -543,6 +620,7 @@ int hf2d_upload_mesh(hf2d_sim* s, const double* bed, const double* bed_tangent, 
       HIPCHK(s, hipMemset(s->mant, 0, bytes));
   }
   s->ws_valid = false;
+  s->list_valid = false;
   return HF2D_OK;
 }
 
@@ -588,6 +666,7 @@ int hf2d_upload_state(hf2d_sim* s, const double* aos, const double* tan_aos) {
     }
   }
   s->ws_valid = false;
+  s->list_valid = false;
   return HF2D_OK;
 }
 
@@ -645,6 +724,7 @@ int hf2d_set_boundaries(hf2d_sim* s, const int32_t kinds[4], const double params
     s->bc_param[k] = params ? params[k] : 0.0;
   }
   s->ws_valid = false;
+  s->list_valid = false;
   return HF2D_OK;
 }
 
@@ -708,6 +788,7 @@ int hf2d_apply_point_sources(hf2d_sim* s, const hf2d_point_source* src, int64_t 
   s->n_src = saved_n;
   s->cap_src = saved_cap;
   s->ws_valid = false;
+  s->list_valid = false;
   return rc;
 }
 
@@ -728,8 +809,11 @@ int hf2d_configure(hf2d_sim* s, const hf2d_config* cfg, const hf2d_point_source*
   if (k > 0 && !src) return fail(s, HF2D_INVALID_ARGUMENT, "null sources");
   int rc = upload_sources(s, src, k);
   if (rc) return rc;
+  rc = upload_pins(s, src, k);
+  if (rc) return rc;
   s->cfg = *cfg;
   s->ws_valid = false;
+  s->list_valid = false;
   return HF2D_OK;
 }
 

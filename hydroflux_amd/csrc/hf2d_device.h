@@ -64,7 +64,11 @@ template <> struct Shape<float> {
   static constexpr int TW = HF_F32_TILE_W, TH = HF_F32_TILE_H, NT = HF_F32_THREADS;
 };
 
-__device__ inline void block_max_to(double sx, double sy, double (*red)[16], unsigned long long* dst) {
+// Workgroup maxima of the CFL wave speeds into dst (64-bit atomic max of
+// the non-negative doubles' bit patterns). With wet_dst (then red has 3
+// rows), also whether any thread saw a wet output cell, one byte.
+__device__ inline void block_max_to(double sx, double sy, double (*red)[16], unsigned long long* dst,
+                                    bool wet = false, unsigned char* wet_dst = nullptr) {
   const int tid = threadIdx.x;
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
@@ -72,19 +76,24 @@ __device__ inline void block_max_to(double sx, double sy, double (*red)[16], uns
     sy = __builtin_fmax(sy, __shfl_xor(sy, off));
   }
   const int wave = tid >> 6, lane = tid & 63;
+  const bool wave_wet = wet_dst ? __any(wet ? 1 : 0) != 0 : false;
   if (lane == 0) {
     red[0][wave] = sx;
     red[1][wave] = sy;
+    if (wet_dst) red[2][wave] = wave_wet ? 1.0 : 0.0;
   }
   __syncthreads();
   if (tid == 0) {
     const int nw = (blockDim.x + 63) >> 6;
-    for (int w = 1; w < nw; ++w) {
-      sx = __builtin_fmax(sx, red[0][w]);
-      sy = __builtin_fmax(sy, red[1][w]);
+    bool w = wave_wet;
+    for (int k = 1; k < nw; ++k) {
+      sx = __builtin_fmax(sx, red[0][k]);
+      sy = __builtin_fmax(sy, red[1][k]);
+      if (wet_dst) w = w || red[2][k] != 0.0;
     }
     atomicMax(dst + 0, hfm::as_u64(sx));
     atomicMax(dst + 1, hfm::as_u64(sy));
+    if (wet_dst) *wet_dst = w ? 1 : 0;
   }
 }
 
@@ -95,7 +104,14 @@ __global__ __launch_bounds__(Shape<T>::NT) void stage_kernel(StageArgs<T> a) {
   __shared__ typename St::S s;
   double dt;
   if (!hf::stage_prologue(a, blockIdx.x == 0 && threadIdx.x == 0, &dt)) return;
-  const unsigned tile = tile_of_block(blockIdx.x, gridDim.x);
+  unsigned tile;
+  if (a.tile_list) {  // active-tile list: positions past its length have no work
+    const unsigned n = (unsigned)*a.n_work;
+    if (blockIdx.x >= n) return;
+    tile = (unsigned)a.tile_list[tile_of_block(blockIdx.x, n)];
+  } else {
+    tile = tile_of_block(blockIdx.x, gridDim.x);
+  }
   const int64_t ty = tile / a.tiles_x, tx = tile % a.tiles_x;
   const int64_t gi0 = ty * TH, gj0 = tx * TW;
   const int tid = threadIdx.x;
@@ -108,11 +124,64 @@ __global__ __launch_bounds__(Shape<T>::NT) void stage_kernel(StageArgs<T> a) {
   St::phase_alpha(s, a, gi0, gj0, tid, NT, dt);
   __syncthreads();
   double wsx = 0.0, wsy = 0.0;
-  St::template phase_update<MODE>(s, a, gi0, gj0, tid, NT, dt, &wsx, &wsy);
+  bool wet = false;
+  St::template phase_update<MODE>(s, a, gi0, gj0, tid, NT, dt, &wsx, &wsy, &wet);
   if (MODE == hf::MODE_FINAL_RK2 && a.wave_speed) {
     Ctrl* c = a.ctrl;
-    block_max_to(wsx, wsy, s.red, &c->ws[c->parity ^ 1][0]);
+    block_max_to(wsx, wsy, s.red, &c->ws[c->parity ^ 1][0], wet, a.tile_wet ? a.tile_wet + tile : nullptr);
   }
+}
+
+// Rebuild the active-tile list for the step about to run (one workgroup of
+// 1024 threads): tiles in row-major order without the ones tile_skipped()
+// leaves out; *n_work = its length. act: per-tile scratch.
+__global__ __launch_bounds__(1024) void compact_kernel(const unsigned char* __restrict__ wet,
+                                                       const unsigned char* __restrict__ pinned,
+                                                       unsigned char* quiet_run, unsigned char* act, int tiles_x,
+                                                       int tiles_y, int* list, int* n_work) {
+  __shared__ int counts[1024];
+  const int n = tiles_x * tiles_y;
+  const int per = (n + 1023) / 1024;
+  const int t0 = (int)threadIdx.x * per;
+  const int t1 = t0 + per < n ? t0 + per : n;
+  int cnt = 0;
+  for (int t = t0; t < t1; ++t) {
+    const bool keep = !hf::tile_skipped(wet, pinned, quiet_run, tiles_x, tiles_y, t % tiles_x, t / tiles_x);
+    act[t] = keep ? 1 : 0;
+    cnt += keep ? 1 : 0;
+  }
+  counts[threadIdx.x] = cnt;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {  // inclusive scan
+    const int v = (int)threadIdx.x >= off ? counts[threadIdx.x - off] : 0;
+    __syncthreads();
+    counts[threadIdx.x] += v;
+    __syncthreads();
+  }
+  int pos = counts[threadIdx.x] - cnt;
+  for (int t = t0; t < t1; ++t)
+    if (act[t]) list[pos++] = t;
+  if (threadIdx.x == 1023) *n_work = counts[1023];
+}
+
+// Per tile, whether a cell of the current state is wet (not h <= H_DRY):
+// the active-tile list's starting point after the state was set by other
+// means than the driver. wet must be zero on entry.
+template <class T>
+__global__ __launch_bounds__(256) void tile_wet_init_kernel(Fld<T> h, int64_t nr, int64_t nc, int tw, int th,
+                                                            int tiles_x, unsigned char* wet) {
+  using R = typename hf::ScalarOf<T>::type;
+  const int64_t n = nr * nc;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+    if (!(hf::val(h.ld(i)) <= R(hf::H_DRY))) {
+      const int64_t gi = i / nc, gj = i % nc;
+      wet[(gi / th) * tiles_x + gj / tw] = 1;  // every writer stores the same byte value
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void fill_bytes_kernel(unsigned char* p, int64_t n, unsigned char v) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) p[i] = v;
 }
 
 // Forward-Euler tail and the standalone source steps: elementwise over
@@ -297,7 +366,8 @@ __global__ __launch_bounds__(NT) void batch_stage_kernel(const StageArgs<T>* __r
     St::phase_alpha(s, a, gi0, gj0, tid, NT, dt);
     __syncthreads();
     double wsx = 0.0, wsy = 0.0;
-    St::template phase_update<MODE>(s, a, gi0, gj0, tid, NT, dt, &wsx, &wsy);
+    bool wet = false;
+    St::template phase_update<MODE>(s, a, gi0, gj0, tid, NT, dt, &wsx, &wsy, &wet);
     if (MODE == hf::MODE_FINAL_RK2 && a.wave_speed) {
       Ctrl* c = a.ctrl;
       block_max_to(wsx, wsy, s.red, &c->ws[c->parity ^ 1][0]);

@@ -127,6 +127,12 @@ template <class T> struct StageArgs {
   const PointSrc* srcs;
   int wave_speed;  // reduce the output's wave speeds into ctrl->ws[parity ^ 1]
   int tiles_x, tiles_y;
+  // Compacted active-tile list (null: every tile). Block b of a launch
+  // takes list position b of the first *n_work; tile_wet (final stage,
+  // nullable) receives per tile whether an output cell is wet.
+  const int* tile_list;
+  const int* n_work;
+  unsigned char* tile_wet;
   // FP32 precision form: bed = datum[block] (FP64, one per 2^datum_log2
   // square block of cells) + zdev (FP32 deviation); `z` is unused then.
   const double* datum;
@@ -342,7 +348,7 @@ template <class T, int TW, int TH> struct TileSmem {
   T alpha[NA];
   unsigned char dry[N4];
   unsigned char nd[N3];  // bit 0: a dry cell within +-2 along x; bit 1: along y
-  double red[2][16];
+  double red[3][16];
 
   static HF_HD int i3(int li, int lj) {
     HF_BOUNDS_CHECK(li >= -3 && li < TH + 3 && lj >= -3 && lj < TW + 3);
@@ -895,10 +901,11 @@ template <class T, int TW, int TH> struct Stage {
 
   // Phase 5: FV update with the explicit bed-slope source
   // (update.rs:1071-1210), then the stage epilogue: write U^(1), or the
-  // SSP-RK2 combine (update.rs:1272-1286) + forcing + CFL wave speeds.
+  // SSP-RK2 combine (update.rs:1272-1286) + forcing + CFL wave speeds;
+  // *wet is set when an output cell is wet (not h <= H_DRY).
   template <int MODE>
   static HF_HD void phase_update(S& s, const StageArgs<T>& a, int64_t gi0, int64_t gj0, int tid, int nt,
-                                 double dt, double* wsx, double* wsy) {
+                                 double dt, double* wsx, double* wsy, bool* wet) {
     const T Z = cst<T>(0.0);
     const R dt_dx = R(dt / a.dx), dt_dy = R(dt / a.dy), rdt = R(dt);
     for (int k = tid; k < TW * TH; k += nt) {
@@ -1009,6 +1016,7 @@ template <class T, int TW, int TH> struct Stage {
           nhv = Z;
         }
         forcing(a, gi, gj, dt, &nh, &nhu, &nhv);
+        if (!(val(nh) <= R(H_DRY))) *wet = true;
         if (a.wave_speed) {
           T zn4[4];
           zn4[SIDE_N] = (a.nr >= 2 && gi == 0) ? s.z[S::i3(li + 1, lj)] : s.z[c3];
@@ -1025,6 +1033,53 @@ template <class T, int TW, int TH> struct Stage {
     }
   }
 };
+
+// ------------------------------------------------------ active-tile list
+//
+// The reference skips dry cells inside a step: interior dry-dry faces get a
+// zero flux and an interior dry cell with four dry neighbours keeps h and
+// drops its momentum (update.rs:871-900, 1060-1090). At tile granularity
+// this becomes a list of the tiles a step must touch, rebuilt every step.
+//
+// A tile is quiet for a step when no cell of it or of its 8 neighbour tiles
+// is wet (h > H_DRY, or NaN) in U^n, no cell within 2 of it lies on the
+// domain edge, and it holds no point source. Then every cell of the tile
+// and of its 1-cell ring is an interior isolated-dry cell in stage 1, so
+// U^(1) = (h, 0, 0) there, and the same holds in stage 2; the RK combine
+// returns h exactly ((h + h) * 0.5), the floor sets (max(h, 0), 0, 0), and
+// without rain and with friction off for dry cells nothing else changes.
+// After two quiet steps both state buffers hold that fixed point bit for
+// bit (max is idempotent), so from the third quiet step on the tile is left
+// out of the list and untouched; its cells add nothing to the CFL maxima.
+// The driver uses the list only where this argument holds: f64 and f32
+// state (a Dual tangent at h == 0 halves at every max(h, 0) tie, so it is
+// not a fixed point), SSP-RK2, no rain, friction dry tolerance >= H_DRY.
+
+// Tiles that are never quiet: within 2 cells of the domain edge.
+HF_HD bool tile_near_edge(int64_t nr, int64_t nc, int tw, int th, int64_t tx, int64_t ty) {
+  const int64_t gi0 = ty * th, gj0 = tx * tw;
+  return gi0 < 3 || gj0 < 3 || gi0 + th > nr - 3 || gj0 + tw > nc - 3;
+}
+
+// Decide whether tile (tx, ty) is left out of this step, and advance its
+// count of consecutive quiet steps (saturating at 2). wet: per tile, a wet
+// cell in U^n; pinned: per tile, never quiet (edge or point source).
+HF_HD bool tile_skipped(const unsigned char* wet, const unsigned char* pinned, unsigned char* quiet_run, int tiles_x,
+                        int tiles_y, int tx, int ty) {
+  const int t = ty * tiles_x + tx;
+  bool quiet = pinned[t] == 0;
+  for (int dy = -1; dy <= 1 && quiet; ++dy)
+    for (int dx = -1; dx <= 1; ++dx) {
+      const int y = ty + dy, x = tx + dx;
+      if (y >= 0 && y < tiles_y && x >= 0 && x < tiles_x && wet[y * tiles_x + x]) {
+        quiet = false;
+        break;
+      }
+    }
+  const unsigned char r = quiet_run[t];
+  quiet_run[t] = quiet ? (unsigned char)(r >= 2 ? 2 : r + 1) : (unsigned char)0;
+  return quiet && r >= 2;
+}
 
 // dt of the step about to run, from the device clock: the CFL bound of
 // the current state (update.rs:170-208), capped by max_dt and dt_cap

@@ -58,6 +58,17 @@ enum hf2d_bc_kind {
 /* Integrator (sim.rs:37-46). */
 enum hf2d_integrator { HF2D_FORWARD_EULER = 0, HF2D_SSPRK2 = 1 };
 
+/* hf2d_config.options. By default the fused driver (hf2d_step, run_until,
+ * run_steps, enqueue_steps) runs each step over a compacted list of active
+ * tiles, rebuilt every step on the device: the reference's cell-mask early
+ * skip (update.rs:871-900, 1060-1090) at tile granularity. Tiles that have
+ * been dry, with a dry neighbourhood, for two steps are left out; their
+ * state is a fixed point of the step, so the results are the same bits.
+ * Used for f64 and f32 grids without rain and with friction off on dry
+ * cells (friction_dry_tol >= 1e-6); otherwise every tile runs. This flag
+ * turns the list off (for timing comparisons). */
+enum hf2d_option { HF2D_OPT_NO_ACTIVE_TILES = 1 };
+
 /* PointSource (source.rs:36-44). */
 typedef struct hf2d_point_source {
   int64_t row;
@@ -75,7 +86,7 @@ typedef struct hf2d_config {
   int32_t cfl_with_bcs;    /* 1: cfl_time_step_with_bcs (Simulation); 0: cfl_time_step */
   double max_dt;           /* > 0; default 60 s */
   int32_t friction;        /* apply manning_friction_step after the integrator */
-  int32_t reserved;
+  int32_t options;         /* bit flags, HF2D_OPT_*; 0 = defaults */
   double friction_dry_tol; /* default H_DRY = 1e-6 */
   int64_t max_steps;       /* run_until budget; default 1e7 */
   double rain_rate;        /* apply_rain rate [m/s] after friction; 0 disables */

@@ -182,3 +182,28 @@ def bumpy_wet_dry(nr=37, nc=53, seed=7, dry_frac=0.35):
     st[film, 1] = 1e-6
     man = rng.choice([0.0, 0.03, 0.05, 0.08], size=(nr, nc))
     return Case(f"bumpy_{nr}x{nc}", nr, nc, 0.7, 1.3, bed, man, st, (WALL, TRANS, WALL, TRANS))
+
+
+def mostly_dry(nr=44, nc=60, seed=5, west=DISCHARGE):
+    """Dry-domain mix for the dry-tile skip: a dry rough bed with one pond,
+    a Discharge (or Depth) inflow on the west edge, dry cells carrying stale momentum,
+    depths exactly at and just above H_DRY, negative zeros, and films that
+    a source or the pond reaches only after some steps."""
+    rng = np.random.default_rng(seed)
+    y, x = np.mgrid[0:nr, 0:nc]
+    bed = 0.02 * x + 0.3 * np.sin(y / 4.0) + 0.05 * rng.standard_normal((nr, nc))
+    st = np.zeros((nr, nc, 3))
+    pond = (x - 0.7 * nc) ** 2 + (y - 0.3 * nr) ** 2 < 20
+    st[pond, 0] = 0.4
+    st[pond, 1] = 0.05
+    dry = ~pond
+    stale = dry & (rng.random((nr, nc)) < 0.2)
+    st[stale, 0] = 3e-7
+    st[stale, 1] = 2e-7 * rng.standard_normal(np.count_nonzero(stale))
+    st[stale, 2] = -1e-7
+    st[nr // 2, nc // 2, 0] = 1e-6                                 # exactly H_DRY: dry
+    st[(nr // 2 + 7) % nr, (nc // 2 - 9) % nc, 0] = 1.000001e-6   # just wet: keeps its tiles active
+    st[3 % nr, 40 % nc, :] = -0.0
+    man = rng.choice([0.0, 0.03, 0.05], size=(nr, nc))
+    return Case(f"mostly_dry_{nr}x{nc}", nr, nc, 1.0, 1.0, bed, man, st, (WALL, WALL, west, TRANS),
+                (0.0, 0.0, 0.05, 0.0))

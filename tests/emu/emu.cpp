@@ -42,7 +42,15 @@ struct Emu {
   int64_t max_steps = 10000000;
   std::vector<PointSrc> srcs;
   double tmp_rain = 0.0, tmp_tol = 0.0;
+  // active-tile list, as the device driver keeps it
+  int list_on = 1;
+  bool list_valid = false;
+  std::vector<unsigned char> wet, run, pin;
+  std::vector<int> list;
 };
+
+// Tiles the active-tile list left out (all emulator steps).
+int64_t g_tiles_skipped = 0;
 
 template <class T> Fld<T> fld(const std::vector<double>& v, const std::vector<double>& d);
 template <> Fld<double> fld<double>(const std::vector<double>& v, const std::vector<double>&) {
@@ -89,31 +97,35 @@ template <class T> StageArgs<T> args(Emu& e, int in, int out, int un, int tw, in
   return a;
 }
 
-// One launch of the stage kernel: every tile, every phase, every thread.
+// One launch of the stage kernel: every tile (or every tile of the active
+// list), every phase, every thread.
 template <class T, int TW, int TH>
 void launch(const StageArgs<T>& a, int mode, unsigned long long* dst) {
   using St = Stage<T, TW, TH>;
   std::unique_ptr<typename St::S> s(new typename St::S());
   const int NT = 256;
-  const int64_t nb = (int64_t)a.tiles_x * a.tiles_y;
+  const int64_t nb = a.tile_list ? *a.n_work : (int64_t)a.tiles_x * a.tiles_y;
   double gx = 0.0, gy = 0.0;
+  double dt;
+  if (!stage_prologue(a, true, &dt)) return;  // workgroup 0's prologue
   for (int64_t b = 0; b < nb; ++b) {
-    double dt;
-    if (!stage_prologue(a, b == 0, &dt)) return;
-    const int64_t gi0 = (b / a.tiles_x) * TH, gj0 = (b % a.tiles_x) * TW;
+    const int64_t tile = a.tile_list ? a.tile_list[b] : b;
+    const int64_t gi0 = (tile / a.tiles_x) * TH, gj0 = (tile % a.tiles_x) * TW;
     for (int t = 0; t < NT; ++t) St::phase_load(*s, a, gi0, gj0, t, NT);
     for (int t = 0; t < NT; ++t) St::phase_diffs(*s, a, gi0, gj0, t, NT);
     for (int t = 0; t < NT; ++t) St::phase_faces(*s, a, gi0, gj0, t, NT);
     for (int t = 0; t < NT; ++t) St::phase_alpha(*s, a, gi0, gj0, t, NT, dt);
+    bool wet = false;
     for (int t = 0; t < NT; ++t) {
       double wx = 0.0, wy = 0.0;
       if (mode == MODE_STAGE)
-        St::template phase_update<MODE_STAGE>(*s, a, gi0, gj0, t, NT, dt, &wx, &wy);
+        St::template phase_update<MODE_STAGE>(*s, a, gi0, gj0, t, NT, dt, &wx, &wy, &wet);
       else
-        St::template phase_update<MODE_FINAL_RK2>(*s, a, gi0, gj0, t, NT, dt, &wx, &wy);
+        St::template phase_update<MODE_FINAL_RK2>(*s, a, gi0, gj0, t, NT, dt, &wx, &wy, &wet);
       gx = rmax(gx, wx);
       gy = rmax(gy, wy);
     }
+    if (a.tile_wet && mode == MODE_FINAL_RK2) a.tile_wet[tile] = wet ? 1 : 0;
   }
   if (dst && mode == MODE_FINAL_RK2 && a.wave_speed) {
     if (hfm::as_u64(gx) > dst[0]) dst[0] = hfm::as_u64(gx);
@@ -177,6 +189,42 @@ template <class T, int TW, int TH> void host_dt_step(Emu& e, double dt, bool rk2
   }
 }
 
+// The device driver's active-tile list (hf2d.hip enqueue_compact), on the
+// host: same usability rule, same per-tile decision (tile_skipped).
+template <class T, int TW, int TH> bool build_list(Emu& e, StageArgs<T>& a, StageArgs<T>& b, int64_t* n_work) {
+  const bool usable = !e.dual_kind && e.integrator == 1 && e.rain == 0.0 && (!e.friction || e.fdt >= H_DRY) &&
+                      e.list_on;
+  if (!usable) {
+    e.list_valid = false;
+    return false;
+  }
+  const int tx = a.tiles_x, ty = a.tiles_y;
+  const size_t nt = (size_t)tx * ty;
+  e.pin.assign(nt, 0);
+  for (int y = 0; y < ty; ++y)
+    for (int x = 0; x < tx; ++x) e.pin[(size_t)y * tx + x] = tile_near_edge(e.nr, e.nc, TW, TH, x, y);
+  for (const PointSrc& p : e.srcs) e.pin[(size_t)(p.row / TH) * tx + (size_t)(p.col / TW)] = 1;
+  if (!e.list_valid || e.wet.size() != nt) {
+    e.wet.assign(nt, 0);
+    e.run.assign(nt, 0);
+    for (int64_t i = 0; i < e.nr * e.nc; ++i)
+      if (!(e.st[e.cur][0][(size_t)i] <= H_DRY)) e.wet[(size_t)((i / e.nc) / TH * tx + (i % e.nc) / TW)] = 1;
+    e.list_valid = true;
+  }
+  e.list.clear();
+  for (int y = 0; y < ty; ++y)
+    for (int x = 0; x < tx; ++x) {
+      if (tile_skipped(e.wet.data(), e.pin.data(), e.run.data(), tx, ty, x, y))
+        ++g_tiles_skipped;
+      else
+        e.list.push_back(y * tx + x);
+    }
+  *n_work = (int64_t)e.list.size();
+  a.tile_list = b.tile_list = e.list.data();
+  b.tile_wet = e.wet.data();
+  return true;
+}
+
 template <class T, int TW, int TH> void driver_step(Emu& e) {
   const int cur = e.cur, oth = 1 - cur;
   StageArgs<T> a = args<T>(e, cur, oth, cur, TW, TH);
@@ -190,6 +238,12 @@ template <class T, int TW, int TH> void driver_step(Emu& e) {
   b.rain_rate = e.rain;
   b.n_src = (int)e.srcs.size();
   b.wave_speed = 1;
+  int64_t n_work64 = 0;
+  int n_work = 0;
+  if (build_list<T, TW, TH>(e, a, b, &n_work64)) {
+    n_work = (int)n_work64;
+    a.n_work = b.n_work = &n_work;
+  }
   launch<T, TW, TH>(a, MODE_STAGE, nullptr);
   if (e.integrator == 1) {
     launch<T, TW, TH>(b, MODE_FINAL_RK2, e.ctrl.ws[e.ctrl.parity ^ 1]);
@@ -274,6 +328,7 @@ template <class T> int call(Emu& e, int op, double x, int64_t n, bool flag) {
 }
 
 int dispatch(Emu& e, int op, double x, int64_t n, bool flag) {
+  if (op != 2 && op != 3) e.list_valid = false;  // the state changed outside the driver
   return e.dual_kind ? call<dual>(e, op, x, n, flag) : call<double>(e, op, x, n, flag);
 }
 
@@ -310,6 +365,7 @@ void emu_destroy(void* p) { delete static_cast<Emu*>(p); }
 
 void emu_set_mesh(void* p, const double* bed, const double* bt, const double* man, const double* mt) {
   Emu* e = static_cast<Emu*>(p);
+  e->list_valid = false;
   const size_t n = e->z.size();
   for (size_t k = 0; k < n; ++k) {
     e->z[k] = bed[k];
@@ -326,6 +382,7 @@ void emu_set_mesh(void* p, const double* bed, const double* bt, const double* ma
 
 void emu_set_state(void* p, const double* aos, const double* tan) {
   Emu* e = static_cast<Emu*>(p);
+  e->list_valid = false;
   const size_t n = e->z.size();
   for (int f = 0; f < 3; ++f)
     for (size_t k = 0; k < n; ++k) {
@@ -346,6 +403,7 @@ void emu_get_state(void* p, double* aos, double* tan) {
 
 void emu_set_boundaries(void* p, const int* kinds, const double* params) {
   Emu* e = static_cast<Emu*>(p);
+  e->list_valid = false;
   for (int k = 0; k < 4; ++k) {
     e->bc_kind[k] = kinds[k];
     e->bc_param[k] = params[k];
@@ -356,6 +414,7 @@ void emu_configure(void* p, double cfl, int integrator, int with_bcs, double max
                    int64_t max_steps, double rain, const int64_t* rows, const int64_t* cols, const double* q,
                    int64_t k) {
   Emu* e = static_cast<Emu*>(p);
+  e->list_valid = false;
   e->cfl = cfl;
   e->integrator = integrator;
   e->with_bcs = with_bcs;
@@ -367,6 +426,9 @@ void emu_configure(void* p, double cfl, int integrator, int with_bcs, double max
   e->srcs.clear();
   for (int64_t s = 0; s < k; ++s) e->srcs.push_back(PointSrc{rows[s], cols[s], q[s]});
 }
+
+void emu_set_active_tiles(void* p, int on) { static_cast<Emu*>(p)->list_on = on; }
+int64_t emu_tiles_skipped() { return g_tiles_skipped; }
 
 void emu_ssprk2_step(void* p, double dt) { dispatch(*static_cast<Emu*>(p), 0, dt, 0, false); }
 void emu_forward_euler_step(void* p, double dt) { dispatch(*static_cast<Emu*>(p), 1, dt, 0, false); }

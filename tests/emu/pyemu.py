@@ -44,6 +44,9 @@ def lib(asan: bool = False):
         L.emu_manning_friction_step.argtypes = [C.c_void_p, C.c_double, C.c_double]
         L.emu_apply_rain.argtypes = [C.c_void_p, C.c_double, C.c_double]
         L.emu_apply_point_sources.argtypes = [C.c_void_p, _I64, _I64, _D, C.c_int64, C.c_double]
+        L.emu_set_active_tiles.argtypes = [C.c_void_p, C.c_int]
+        L.emu_tiles_skipped.restype = C.c_int64
+        L.emu_tiles_skipped.argtypes = []
         _LIBS[name] = L
     return _LIBS[name]
 
@@ -63,6 +66,12 @@ class EmuSim:
         tw, th = tile if tile else ((16, 16) if dual else (32, 16))
         self._L = lib(asan)
         self._h = self._L.emu_create(self.n_rows, self.n_cols, float(dx), float(dy), int(dual), tw, th)
+
+    def set_active_tiles(self, on: bool):
+        self._L.emu_set_active_tiles(self._h, 1 if on else 0)
+
+    def tiles_skipped(self) -> int:
+        return int(self._L.emu_tiles_skipped())
 
     def close(self):
         if self._h:

@@ -97,3 +97,67 @@ def test_driver_errors_match_oracle():
         s.configure(max_steps=3)
         assert s.run_until(100.0) == 4
         assert s.time()[1] == 3
+
+
+@pytest.mark.parametrize("tile", [(8, 4), (4, 8), (16, 8), (32, 16)], ids=lambda t: f"{t[0]}x{t[1]}")
+@pytest.mark.parametrize("west", [cases.DISCHARGE, cases.DEPTH], ids=["discharge", "depth"])
+def test_active_tile_list_bitwise(tile, west):
+    """The compacted active-tile list leaves settled dry tiles out; results
+    stay the oracle's bits and equal a run over every tile."""
+    c = cases.mostly_dry(60, 84, 5, west=west)
+    o = c.load(OracleSim(c.nr, c.nc, c.dx, c.dy))
+    on = c.load(EmuSim(c.nr, c.nc, c.dx, c.dy, tile=tile))
+    off = c.load(EmuSim(c.nr, c.nc, c.dx, c.dy, tile=tile))
+    off.set_active_tiles(False)
+    src = [(c.nr - 6, 8, 0.02), (30, 50, 0.01)]
+    for s in (o, on, off):
+        s.configure(cfl=0.4, max_dt=0.2, sources=src)
+    before = on.tiles_skipped()
+    assert on.run_steps(30) == 0
+    skipped = on.tiles_skipped() - before
+    assert off.run_steps(30) == 0
+    assert on.tiles_skipped() - before == skipped  # the run over every tile skipped nothing
+    # the state set from the host mid-run restarts the list from its wet flags
+    st = on.get_state()
+    st[40, 20] = (0.3, 0.0, 0.0)
+    on.set_state(st)
+    off.set_state(st)
+    o.run_steps(30)
+    o.set_state(st)
+    for s in (o, on, off):
+        s.run_steps(25)
+    if tile != (32, 16):
+        assert skipped > 0
+    assert o.time() == on.time() == off.time()
+    assert np.array_equal(on.get_state(), off.get_state())
+    assert np.array_equal(o.get_state(), on.get_state())
+
+
+def test_active_tile_list_not_used_with_rain():
+    c = cases.mostly_dry(40, 48, 2)
+    e = c.load(EmuSim(c.nr, c.nc, c.dx, c.dy, tile=(8, 4)))
+    e.configure(cfl=0.4, max_dt=0.2, rain_rate=1e-7)
+    before = e.tiles_skipped()
+    e.run_steps(10)
+    assert e.tiles_skipped() == before
+
+
+def test_active_tile_list_dual_runs_every_tile():
+    """Dual: a tangent at h == 0 halves at each max(h, 0) tie, so no tile is
+    a fixed point and the list is not used; the results are the oracle's."""
+    c = cases.mostly_dry(36, 40, 9)
+    rng = np.random.default_rng(4)
+    stt = 0.1 * rng.standard_normal((c.nr, c.nc, 3))
+    o = c.load(OracleSim(c.nr, c.nc, c.dx, c.dy, dual=True), manning_tangent=c.manning, state_tangent=stt)
+    e = c.load(EmuSim(c.nr, c.nc, c.dx, c.dy, dual=True, tile=(8, 4)), manning_tangent=c.manning,
+               state_tangent=stt)
+    for s in (o, e):
+        s.configure(cfl=0.4, max_dt=0.2, sources=[(30, 5, 0.01)])
+    before = e.tiles_skipped()
+    o.run_steps(25)
+    e.run_steps(25)
+    assert e.tiles_skipped() == before
+    ov, ot = o.get_state()
+    ev, et = e.get_state()
+    # the Discharge-on-dry ghost gives NaN tangents along the inflow edge in both
+    assert np.array_equal(ov, ev) and np.array_equal(ot, et, equal_nan=True)
