@@ -206,7 +206,7 @@ HF_HD void ghost_state(int kind, double param, int side, T h, T hu, T hv, T zc, 
       // bed_slope_x/y (geometry.rs:131-146), magnitude only
       double slope = have_next ? __builtin_fabs((double)val(zc - zn) / d) : 0.0;
       if (slope > DRY_INFLOW_SLOPE_THRESHOLD) {
-        T hn = powf_gen(man * R(__builtin_fabs(param) / __builtin_sqrt(slope)), 3.0 / 5.0);
+        T hn = pow35(man * R(__builtin_fabs(param) / __builtin_sqrt(slope)));
         *gh = rmin(hn, cst<T>(DRY_INFLOW_H_MAX));
       }
     }

@@ -1,4 +1,5 @@
-// Deterministic real powers used by the Manning friction step.
+// Deterministic real powers used by the Manning friction step and the
+// Discharge-onto-dry ghost.
 //
 // The reference evaluates `h.powf(4.0 / 3.0)` (solver-2d/src/source.rs:161)
 // and, for forward-mode tangents, `n * v.powf(n - 1.0) * dv`
@@ -159,6 +160,112 @@ HF_HD double pow_1_3m(double x) {
   double L = ln_pos(x);
   double lo = d - (y * HFM_POW_EPS) * L;
   return scale2(y + lo, k);
+}
+
+// ------------------------------------------------------------ x^(3/5)
+//
+// The Discharge-onto-dry ghost depth (n |q| / sqrt(S))^(3/5)
+// (solver-2d/src/boundary.rs:203-216) and, for its Dual tangent,
+// pow(x, 3.0/5.0 - 1.0). The double nearest 3/5 is e = 3/5 - eps5 with
+// eps5 = 2^-53 / 5, and e - 1 = -2/5 - eps5 exactly (Sterbenz), so the
+// routines evaluate x^(3/5) resp. x^(-2/5) times x^(-eps5) = 1 - eps5 ln x.
+//
+// Method: x = m * 2^(5k), m in [1, 32). z ~ m^(-1/5) from a quadratic seed
+// on the mantissa and four division-free Newton steps
+// z <- z + z (1 - m z^5) / 5; y = m z^4 ~ m^(1/5) then gets one
+// double-double correction d = (m - y^5) z^4 / 5 with m - y^5 formed from
+// FMA two-products. (y + d)^3 and (y + d)^3 / m (one remainder-corrected
+// division) give m^(3/5) and m^(-2/5) to ~100 bits before the final
+// rounding.
+
+#define HFM_POW5_EPS 0x1.999999999999ap-56  // 3/5 - fl(3/5) = 2^-53 / 5
+
+// Reduce finite x > 0 to m = f * 2^r in [1, 32) (f in [1, 2), r in 0..4)
+// and k with x = m * 2^(5k).
+HF_HD void split_fifth(double x, double* m, double* f_out, int* r_out, int* k) {
+  double f;
+  int E;
+  split_pos(x, &f, &E);
+  int kk = (E >= 0) ? (E / 5) : -((-E + 4) / 5);
+  int r = E - 5 * kk;  // 0..4
+  *m = f * (double)(1 << r);
+  *f_out = f;
+  *r_out = r;
+  *k = kk;
+}
+
+// Fifth root of m = f * 2^r as an unevaluated pair (y, d).
+HF_HD void fifth_root_dd(double m, double f, int r, double* y_out, double* d_out) {
+  double s;  // 2^(-r/5)
+  switch (r) {
+    case 0: s = 1.0; break;
+    case 1: s = 0.8705505632961241; break;
+    case 2: s = 0.757858283255199; break;
+    case 3: s = 0.6597539553864471; break;
+    default: s = 0.5743491774985174; break;
+  }
+  double z = ((0.0521304 * f - 0.28315556) * f + 1.22947226) * s;
+  for (int it = 0; it < 4; ++it) {
+    double t = z * z;
+    t = t * t;
+    t = t * z;
+    t = m * t;
+    double e = 1.0 - t;
+    z = z + z * (e * 0.2);
+  }
+  double z2 = z * z;
+  double z4 = z2 * z2;
+  double y = m * z4;
+  // m - y^5 from two-products: y^2 = a + ea, a^2 = b + eb, b y = c + ec,
+  // y^5 = c + ec + (eb + 2 a ea) y up to terms below 2^-100 m.
+  double a = y * y;
+  double ea = __builtin_fma(y, y, -a);
+  double b = a * a;
+  double eb = __builtin_fma(a, a, -b);
+  double c = b * y;
+  double ec = __builtin_fma(b, y, -c);
+  double res = (m - c) - (ec + (eb + 2.0 * a * ea) * y);
+  *y_out = y;
+  *d_out = res * z4 * 0.2;
+}
+
+// (y + d)^3 as hi + lo.
+HF_HD void cube_dd(double y, double d, double* hi, double* lo) {
+  double a = y * y;
+  double ea = __builtin_fma(y, y, -a);
+  double b = a * y;
+  double eb = __builtin_fma(a, y, -b);
+  *hi = b;
+  *lo = eb + ea * y + 3.0 * a * d;
+}
+
+// pow(x, 3.0/5.0) for the double exponent 3.0/5.0.
+HF_HD double pow_3_5(double x) {
+  if (!(x > 0.0)) return (x == 0.0) ? 0.0 : (x - x) / (x - x);  // 0 or NaN
+  if (x > 1.7976931348623157e308) return x;                      // +inf
+  double m, f, y, d, hi, lo;
+  int r, k;
+  split_fifth(x, &m, &f, &r, &k);
+  fifth_root_dd(m, f, r, &y, &d);
+  cube_dd(y, d, &hi, &lo);
+  lo = lo - (hi * HFM_POW5_EPS) * ln_pos(x);
+  return scale2(hi + lo, 3 * k);
+}
+
+// pow(x, 3.0/5.0 - 1.0), the exponent the Dual tangent of powf(3/5) uses.
+HF_HD double pow_m2_5(double x) {
+  if (!(x > 0.0)) return (x == 0.0) ? __builtin_inf() : (x - x) / (x - x);  // pow(0, y < 0) = inf
+  if (x > 1.7976931348623157e308) return 0.0;
+  double m, f, y, d, hi, lo;
+  int r, k;
+  split_fifth(x, &m, &f, &r, &k);
+  fifth_root_dd(m, f, r, &y, &d);
+  cube_dd(y, d, &hi, &lo);
+  double q = hi / m;  // m^(-2/5) = (y + d)^3 / m
+  double rem = __builtin_fma(-q, m, hi);
+  double ql = (rem + lo) / m;
+  ql = ql - (q * HFM_POW5_EPS) * ln_pos(x);
+  return scale2(q + ql, -2 * k);
 }
 
 }  // namespace hfm

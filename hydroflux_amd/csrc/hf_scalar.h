@@ -100,10 +100,13 @@ HF_HD dual pow43(dual x) {
 }
 HF_HD float pow43(float x) { return x * cbrtf(x); }
 
-// General x.powf(n) for the Discharge-on-dry ghost depth (boundary.rs:212),
-// the only other power on the path.
-HF_HD double powf_gen(double x, double n) { return pow(x, n); }
-HF_HD dual powf_gen(dual x, double n) { return {pow(x.v, n), n * pow(x.v, n - 1.0) * x.d}; }
-HF_HD float powf_gen(float x, double n) { return powf(x, (float)n); }
+// x.powf(3.0 / 5.0) for the Discharge-on-dry ghost depth (boundary.rs:212),
+// the only other power on the path, with the deterministic routines.
+HF_HD double pow35(double x) { return hfm::pow_3_5(x); }
+HF_HD dual pow35(dual x) {
+  const double n = 3.0 / 5.0;
+  return {hfm::pow_3_5(x.v), n * hfm::pow_m2_5(x.v) * x.d};
+}
+HF_HD float pow35(float x) { return powf(x, 0.6f); }
 
 }  // namespace hf

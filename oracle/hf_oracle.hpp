@@ -64,6 +64,8 @@ inline double powf64(double x, double n) {
   if (pow_mode() == 1) {
     if (n == 4.0 / 3.0) return hfm::pow_4_3(x);
     if (n == 4.0 / 3.0 - 1.0) return hfm::pow_1_3m(x);
+    if (n == 3.0 / 5.0) return hfm::pow_3_5(x);
+    if (n == 3.0 / 5.0 - 1.0) return hfm::pow_m2_5(x);
   }
   return std::pow(x, n);
 }

@@ -246,5 +246,7 @@ void hfo_set_time(void* p, double t, int64_t steps) {
 // Correctly rounded powers shared with the kernels, exposed for tests.
 double hfo_pow_4_3(double x) { return hfm::pow_4_3(x); }
 double hfo_pow_1_3m(double x) { return hfm::pow_1_3m(x); }
+double hfo_pow_3_5(double x) { return hfm::pow_3_5(x); }
+double hfo_pow_m2_5(double x) { return hfm::pow_m2_5(x); }
 double hfo_libm_pow(double x, double n) { return std::pow(x, n); }
 }

@@ -140,15 +140,16 @@ def test_driver_with_forcing_bitwise(seed):
     assert np.array_equal(g.get_state(), o.get_state())
 
 
+@pytest.mark.parametrize("west", [cases.DISCHARGE, cases.DEPTH], ids=["discharge", "depth"])
 @pytest.mark.parametrize("kind", ["f64", "dual", "f32"])
-def test_active_tile_list_bitwise(kind):
-    """Mostly dry domain (Depth inflow, pond, stale films, point sources):
-    the compacted active-tile list gives the same bits as a run over every
-    tile, and the oracle's (f64; Dual runs every tile). A Depth edge: the
-    Discharge-onto-dry ghost's pow(., 3/5) is the device libm's, within an
-    ulp of glibc's, not bit-equal."""
-    _say(f"active-tile list {kind}")
-    c = cases.mostly_dry(150, 230, 3, west=cases.DEPTH)
+def test_active_tile_list_bitwise(kind, west):
+    """Mostly dry domain (Discharge or Depth inflow onto dry cells, pond,
+    stale films, point sources): the compacted active-tile list gives the
+    same bits as a run over every tile, and the oracle's (f64 and Dual;
+    Dual runs every tile). The Discharge-onto-dry ghost depth
+    (n |q| / sqrt(S))^(3/5) uses the deterministic power on both sides."""
+    _say(f"active-tile list {kind} west={west}")
+    c = cases.mostly_dry(150, 230, 3, west=west)
     hf = _hf()
     kw = {}
     if kind == "dual":

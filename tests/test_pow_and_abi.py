@@ -52,6 +52,53 @@ def test_pow_special_values():
     assert L.hfo_pow_1_3m(5e-324) == pow(5e-324, 4.0 / 3.0 - 1.0)
 
 
+def _pow35(L):
+    L.hfo_pow_3_5.restype = ctypes.c_double
+    L.hfo_pow_3_5.argtypes = [ctypes.c_double]
+    L.hfo_pow_m2_5.restype = ctypes.c_double
+    L.hfo_pow_m2_5.argtypes = [ctypes.c_double]
+    return ((3.0 / 5.0, L.hfo_pow_3_5), (3.0 / 5.0 - 1.0, L.hfo_pow_m2_5))
+
+
+def test_pow_3_5_routines_are_correctly_rounded():
+    # the Discharge-onto-dry ghost depth x^(3/5) and its Dual tangent power,
+    # against 60-digit decimal arithmetic, over the whole double range
+    rng = np.random.default_rng(13)
+    xs = np.concatenate([10.0 ** rng.uniform(-300, 300, 300), 10.0 ** rng.uniform(-3, 3, 300),
+                         [5e-324, 2.2e-308, 1e-310, 1.7e308, 1.0, 2.0, 31.999999999999996, 1024.0]])
+    L = olib()
+    for e, fn in _pow35(L):
+        for x in xs:
+            got = fn(float(x))
+            ex = _exact_pow(float(x), e)
+            ulp = np.spacing(got)
+            assert abs(Decimal(got) - ex) <= Decimal(ulp) * Decimal("0.5000001"), (x, e)
+
+
+def test_pow_3_5_agrees_with_glibc_within_one_ulp():
+    rng = np.random.default_rng(6)
+    xs = 10.0 ** rng.uniform(-9, 4, 20000)
+    L = olib()
+    for e, fn in _pow35(L):
+        a = np.array([fn(float(x)) for x in xs])
+        b = np.array([L.hfo_libm_pow(float(x), e) for x in xs])
+        assert (np.abs(a - b) / np.spacing(b)).max() <= 1.0
+        assert np.mean(a != b) < 5e-3
+
+
+def test_pow_3_5_special_values_and_exponent_gap():
+    L = olib()
+    (_, p35), (_, pm25) = _pow35(L)
+    assert p35(0.0) == 0.0 and pm25(0.0) == float("inf") and pm25(-0.0) == float("inf")
+    assert p35(float("inf")) == float("inf") and pm25(float("inf")) == 0.0
+    assert np.isnan(p35(-1.0)) and np.isnan(pm25(-2.0)) and np.isnan(p35(float("nan")))
+    # 32^(3/5) = 8 and 32^(-2/5) = 1/4 exactly, but the double exponent is
+    # 3/5 - 2^-53/5, so the powers fall just below: one ulp under 8 and 1/4.
+    assert p35(32.0) == float.fromhex("0x1.fffffffffffffp+2")
+    assert pm25(32.0) == float.fromhex("0x1.fffffffffffffp-3")
+    assert p35(1.0) == 1.0 and pm25(1.0) == 1.0
+
+
 def _declared_symbols():
     txt = open(os.path.join(ROOT, "include", "hydroflux_amd.h")).read()
     return sorted(set(re.findall(r"\b(hf2d_[a-z_0-9]+)\s*\(", txt)))
