@@ -52,6 +52,10 @@ SIGNATURES = {
     "hf2d_apply_point_sources": (C.c_int, [C.c_void_p, C.POINTER(HfPointSource), C.c_int64, C.c_double]),
     "hf2d_configure": (C.c_int, [C.c_void_p, C.POINTER(HfConfig), C.POINTER(HfPointSource), C.c_int64]),
     "hf2d_step": (C.c_int, [C.c_void_p, C.c_double, _D]),
+    "hf2d_set_forcing_series": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, _D, _D, C.c_double, C.c_double,
+                                          C.c_double]),
+    "hf2d_set_gauges": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.c_int64]),
+    "hf2d_read_gauges": (C.c_int, [C.c_void_p, C.c_int64, _D, C.POINTER(C.c_int64)]),
     "hf2d_run_until": (C.c_int, [C.c_void_p, C.c_double]),
     "hf2d_run_steps": (C.c_int, [C.c_void_p, C.c_int64]),
     "hf2d_get_time": (C.c_int, [C.c_void_p, _D, _I64]),

@@ -49,6 +49,15 @@ struct hf2d_sim {
   int* tile_list = nullptr;
   int tiles_x = 0, tiles_y = 0;
   bool list_valid = false;  // tile_wet / tile_run follow the current state
+  // driver forcing series ([0] rain, [1] point sources) and gauges
+  SeriesDesc series_host[2] = {};
+  double* series_buf[2] = {nullptr, nullptr};  // device t[n] then v[n]
+  SeriesDesc* series_dev = nullptr;
+  int64_t* gauge_cells = nullptr;
+  double* gauge_buf = nullptr;
+  long long* gauge_count = nullptr;
+  int n_gauges = 0;
+  long long gauge_cap = 0;
   std::string err;
   // kernel timing
   bool timing = false;
@@ -94,6 +103,7 @@ hf2d_config default_config() {
 // cells, not switched off.
 bool list_usable(const hf2d_sim* s) {
   return s->tile_list != nullptr && s->cfg.integrator == HF2D_SSPRK2 && s->cfg.rain_rate == 0.0 &&
+         s->series_host[0].n == 0 &&
          (!s->cfg.friction || s->cfg.friction_dry_tol >= hf::H_DRY) &&
          (s->cfg.options & HF2D_OPT_NO_ACTIVE_TILES) == 0;
 }
@@ -278,6 +288,8 @@ template <class T> int enqueue_step(hf2d_sim* s) {
   b.rain_rate = s->cfg.rain_rate;
   b.n_src = s->n_src;
   b.wave_speed = 1;
+  a.series = b.series = s->series_dev;
+  b.use_series = 1;
   if (list_usable(s)) {
     int rc = enqueue_compact<T>(s);
     if (rc) return rc;
@@ -324,6 +336,20 @@ template <class T> int enqueue_step(hf2d_sim* s) {
     }
   }
   hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(1), 0, s->stream, s->ctrl);
+  HIPCHK(s, hipGetLastError());
+  if (s->n_gauges > 0) {
+    hipLaunchKernelGGL((gauge_kernel<T>), dim3(1), dim3(64), 0, s->stream, s->ctrl,
+                       fld<T>(s->st[s->cur][0], s->tg[s->cur][0]), s->gauge_cells, s->n_gauges, s->gauge_buf,
+                       s->gauge_count, s->gauge_cap, 0);
+    HIPCHK(s, hipGetLastError());
+  }
+  return HF2D_OK;
+}
+
+template <class T> int record_gauges_now(hf2d_sim* s) {
+  hipLaunchKernelGGL((gauge_kernel<T>), dim3(1), dim3(64), 0, s->stream, s->ctrl,
+                     fld<T>(s->st[s->cur][0], s->tg[s->cur][0]), s->gauge_cells, s->n_gauges, s->gauge_buf,
+                     s->gauge_count, s->gauge_cap, 1);
   HIPCHK(s, hipGetLastError());
   return HF2D_OK;
 }
@@ -563,7 +589,8 @@ int hf2d_destroy(hf2d_sim* s) {
     }
   void* bufs[] = {s->z,        s->zt,       s->man,      s->mant,     s->datum,    s->zdev,
                   s->ctrl,     s->srcs,     s->tile_wet, s->tile_run, s->tile_pin, s->tile_act,
-                  s->tile_list};
+                  s->tile_list, s->series_buf[0], s->series_buf[1], s->series_dev, s->gauge_cells,
+                  s->gauge_buf, s->gauge_count};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   for (hipEvent_t e : s->ev) (void)hipEventDestroy(e);
@@ -814,6 +841,85 @@ int hf2d_configure(hf2d_sim* s, const hf2d_config* cfg, const hf2d_point_source*
   s->cfg = *cfg;
   s->ws_valid = false;
   s->list_valid = false;
+  return HF2D_OK;
+}
+
+int hf2d_set_forcing_series(hf2d_sim* s, int32_t which, int32_t n, const double* t, const double* v,
+                            double time_div, double scale_mul, double scale_div) {
+  if (!valid(s)) return HF2D_INVALID_ARGUMENT;
+  if (which != HF2D_SERIES_RAIN && which != HF2D_SERIES_SOURCES)
+    return fail(s, HF2D_INVALID_ARGUMENT, "unknown forcing series");
+  if (n < 0 || (n > 0 && (!t || !v))) return fail(s, HF2D_INVALID_ARGUMENT, "null series arrays");
+  if (n > 0 && !(time_div > 0.0 && scale_div != 0.0 && std::isfinite(scale_mul)))
+    return fail(s, HF2D_INVALID_CONFIG, "invalid series scaling");
+  for (int32_t k = 0; k < n; ++k)
+    if (!std::isfinite(t[k]) || !std::isfinite(v[k]) || (k > 0 && !(t[k] >= t[k - 1])))
+      return fail(s, HF2D_INVALID_CONFIG, "series times must be finite and non-decreasing");
+  HIPCHK(s, hipStreamSynchronize(s->stream));
+  if (s->series_buf[which]) HIPCHK(s, hipFree(s->series_buf[which]));
+  s->series_buf[which] = nullptr;
+  SeriesDesc d{};
+  if (n > 0) {
+    HIPCHK(s, hipMalloc((void**)&s->series_buf[which], 2 * (size_t)n * sizeof(double)));
+    HIPCHK(s, hipMemcpy(s->series_buf[which], t, (size_t)n * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(s, hipMemcpy(s->series_buf[which] + n, v, (size_t)n * sizeof(double), hipMemcpyHostToDevice));
+    d.n = n;
+    d.time_div = time_div;
+    d.scale_mul = scale_mul;
+    d.scale_div = scale_div;
+    d.t = s->series_buf[which];
+    d.v = s->series_buf[which] + n;
+  }
+  s->series_host[which] = d;
+  if (!s->series_dev) HIPCHK(s, hipMalloc((void**)&s->series_dev, 2 * sizeof(SeriesDesc)));
+  HIPCHK(s, hipMemcpy(s->series_dev, s->series_host, 2 * sizeof(SeriesDesc), hipMemcpyHostToDevice));
+  s->list_valid = false;
+  return HF2D_OK;
+}
+
+int hf2d_set_gauges(hf2d_sim* s, int32_t n, const int64_t* rows, const int64_t* cols, int64_t capacity) {
+  if (!valid(s)) return HF2D_INVALID_ARGUMENT;
+  if (n < 0 || (n > 0 && (!rows || !cols)) || (n > 0 && capacity < 1))
+    return fail(s, HF2D_INVALID_ARGUMENT, "invalid gauge arguments");
+  std::vector<int64_t> cells((size_t)n);
+  for (int32_t k = 0; k < n; ++k) {
+    if (rows[k] < 0 || rows[k] >= s->nr || cols[k] < 0 || cols[k] >= s->nc)
+      return fail(s, HF2D_SHAPE_MISMATCH, "gauge cell outside the grid");
+    cells[(size_t)k] = rows[k] * s->nc + cols[k];
+  }
+  HIPCHK(s, hipStreamSynchronize(s->stream));
+  void* old[] = {s->gauge_cells, s->gauge_buf, s->gauge_count};
+  for (void* p : old)
+    if (p) HIPCHK(s, hipFree(p));
+  s->gauge_cells = nullptr;
+  s->gauge_buf = nullptr;
+  s->gauge_count = nullptr;
+  s->n_gauges = 0;
+  s->gauge_cap = 0;
+  if (n == 0) return HF2D_OK;
+  HIPCHK(s, hipMalloc((void**)&s->gauge_cells, (size_t)n * sizeof(int64_t)));
+  HIPCHK(s, hipMalloc((void**)&s->gauge_buf, (size_t)capacity * (size_t)(n + 1) * sizeof(double)));
+  HIPCHK(s, hipMalloc((void**)&s->gauge_count, sizeof(long long)));
+  HIPCHK(s, hipMemcpy(s->gauge_cells, cells.data(), (size_t)n * sizeof(int64_t), hipMemcpyHostToDevice));
+  const long long zero = 0;
+  HIPCHK(s, hipMemcpy(s->gauge_count, &zero, sizeof zero, hipMemcpyHostToDevice));
+  s->n_gauges = n;
+  s->gauge_cap = capacity;
+  return HF_DISPATCH(s, record_gauges_now, s);
+}
+
+int hf2d_read_gauges(hf2d_sim* s, int64_t max_rows, double* out, int64_t* n_rows) {
+  if (!valid(s) || !n_rows) return HF2D_INVALID_ARGUMENT;
+  long long count = 0;
+  if (s->n_gauges > 0) {
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    HIPCHK(s, hipMemcpy(&count, s->gauge_count, sizeof count, hipMemcpyDeviceToHost));
+  }
+  *n_rows = count;
+  const long long k = count < max_rows ? count : max_rows;
+  if (out && k > 0)
+    HIPCHK(s, hipMemcpy(out, s->gauge_buf, (size_t)k * (size_t)(s->n_gauges + 1) * sizeof(double),
+                        hipMemcpyDeviceToHost));
   return HF2D_OK;
 }
 

@@ -12,6 +12,7 @@ using hf::dual;
 using hf::Fld;
 using hf::FldW;
 using hf::PointSrc;
+using hf::SeriesDesc;
 using hf::StageArgs;
 
 namespace {
@@ -178,6 +179,22 @@ __global__ __launch_bounds__(256) void tile_wet_init_kernel(Fld<T> h, int64_t nr
       wet[(gi / th) * tiles_x + gj / tw] = 1;  // every writer stores the same byte value
     }
   }
+}
+
+// Gauges: append (t, h at each gauge cell) as row *count of buf (rows of
+// n + 1 doubles) after a step that advanced the clock, or unconditionally
+// with force (the initial row). Rows past cap are dropped.
+template <class T>
+__global__ __launch_bounds__(64) void gauge_kernel(const Ctrl* c, Fld<T> h, const int64_t* cells, int n,
+                                                   double* buf, long long* count, long long cap, int force) {
+  if (!force && (c->status != 0 || c->done != 0)) return;
+  const long long row = *count;
+  if (row >= cap) return;
+  double* r = buf + row * (long long)(n + 1);
+  for (int k = (int)threadIdx.x; k <= n; k += (int)blockDim.x)
+    r[k] = k == 0 ? c->t : (double)hf::val(h.ld(cells[k - 1]));
+  __syncthreads();
+  if (threadIdx.x == 0) *count = row + 1;
 }
 
 __global__ __launch_bounds__(256) void fill_bytes_kernel(unsigned char* p, int64_t n, unsigned char v) {

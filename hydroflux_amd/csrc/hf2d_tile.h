@@ -98,7 +98,49 @@ struct Ctrl {
   double t_comp;    // Kahan compensation of t (FP32 precision form)
   int compensated;  // 1: compensated clock; 0: plain t += dt (reference)
   int pad_;
+  // forcing series of the driver, evaluated at t + dt by the first stage:
+  // [0] rain rate [m/s], [1] point-source rate [m^3/s]
+  double series_now[2];
+  int series_on[2];
 };
+
+// The clock after a step of dt, as finalize_kernel advances it.
+HF_HD double clock_after(const Ctrl* c, double dt) {
+  if (c->compensated) return c->t + (dt - c->t_comp);
+  return c->t + dt;  // sim.rs:225
+}
+
+// A piecewise-linear forcing series (hyetograph, hydrograph) of the
+// reference's operational loops: value(t_s) = (interp(t_s / time_div) *
+// scale_mul) / scale_div, with interp as uk_ea_common/mod.rs (interp:
+// clamp at both ends, partition_point, frac = (t - t0) / (t1 - t0),
+// v0 + frac * (v1 - v0)) and the scaling as uk_ea_test4_official.rs
+// (discharge_at, then / inlet cells) and uk_ea_test8a_official.rs
+// (interp_min on t / 60, then * 1e-3 / 3600 for rain, * cellsize for Q).
+struct SeriesDesc {
+  int n;  // 0: no series
+  double time_div, scale_mul, scale_div;
+  const double* t;
+  const double* v;
+};
+
+HF_HD double series_at(const SeriesDesc& s, double t_s) {
+  const double t = t_s / s.time_div;
+  double v;
+  if (t <= s.t[0]) {
+    v = s.v[0];
+  } else if (t >= s.t[s.n - 1]) {
+    v = s.v[s.n - 1];
+  } else {
+    int idx = 0;
+    while (idx < s.n && s.t[idx] <= t) ++idx;
+    const double t0 = s.t[idx - 1], t1 = s.t[idx];
+    const double v0 = s.v[idx - 1], v1 = s.v[idx];
+    const double frac = (t - t0) / (t1 - t0);
+    v = v0 + frac * (v1 - v0);
+  }
+  return (v * s.scale_mul) / s.scale_div;
+}
 
 struct PointSrc {  // source.rs:36-44
   int64_t row, col;
@@ -133,6 +175,11 @@ template <class T> struct StageArgs {
   const int* tile_list;
   const int* n_work;
   unsigned char* tile_wet;
+  // Driver forcing series ([0] rain, [1] point sources; device memory,
+  // nullable): the first stage evaluates them at t + dt into the control
+  // block, and forcing() uses those values when use_series is set.
+  const SeriesDesc* series;
+  int use_series;
   // FP32 precision form: bed = datum[block] (FP64, one per 2^datum_log2
   // square block of cells) + zdev (FP32 deviation); `z` is unused then.
   const double* datum;
@@ -874,8 +921,11 @@ template <class T, int TW, int TH> struct Stage {
         *hv = div(*hv, factor);
       }
     }
-    if (a.rain_rate != 0.0) {
-      const R dh = R(a.rain_rate * dt);
+    const bool rain_series = a.use_series && a.ctrl->series_on[0];
+    const bool src_series = a.use_series && a.ctrl->series_on[1];
+    const double rain = rain_series ? a.ctrl->series_now[0] : a.rain_rate;
+    if (rain != 0.0) {  // source.rs:88-101
+      const R dh = R(rain * dt);
       T nh = *h + dh;
       if (val(nh) <= R(0)) {
         *h = Z;
@@ -889,7 +939,8 @@ template <class T, int TW, int TH> struct Stage {
       const double cell_area = a.dx * a.dy;
       for (int k = 0; k < a.n_src; ++k) {
         if (a.srcs[k].row != gi || a.srcs[k].col != gj) continue;
-        const R dh = R(a.srcs[k].q * dt / cell_area);
+        const double q = src_series ? a.ctrl->series_now[1] : a.srcs[k].q;
+        const R dh = R(q * dt / cell_area);
         *h = rmax(*h + dh, Z);
         if (val(*h) == R(0)) {
           *hu = Z;
@@ -1128,6 +1179,13 @@ HF_HD bool stage_prologue(const StageArgs<T>& a, bool lead, double* dt_out) {
     c->dt = dt;
     c->ws[c->parity ^ 1][0] = 0ULL;
     c->ws[c->parity ^ 1][1] = 0ULL;
+    if (a.series) {
+      const double t_new = clock_after(c, dt);
+      for (int k = 0; k < 2; ++k) {
+        c->series_on[k] = a.series[k].n > 0 ? 1 : 0;
+        if (a.series[k].n > 0) c->series_now[k] = series_at(a.series[k], t_new);
+      }
+    }
   }
   *dt_out = dt;
   return true;

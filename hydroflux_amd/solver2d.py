@@ -225,6 +225,7 @@ class Grid2D:
         self.n_rows, self.n_cols, self.dx, self.dy = int(n_rows), int(n_cols), float(dx), float(dy)
         self.kind = kind
         self.dual = kind == "dual"
+        self._n_gauges = 0
 
     @classmethod
     def for_mesh(cls, mesh: Mesh2D, dual: Optional[bool] = None) -> "Grid2D":
@@ -348,6 +349,40 @@ class Grid2D:
 
     def set_time(self, t: float, steps: int):
         _check(lib().hf2d_set_time(self._h, float(t), int(steps)), self._h)
+
+    def set_forcing_series(self, which: str, times, values, time_div=1.0, scale_mul=1.0, scale_div=1.0):
+        """Piecewise-linear forcing evaluated on the device at t + dt of every
+        driver step: value = (interp(t / time_div) * scale_mul) / scale_div.
+        which = "rain" (rate, m/s) or "sources" (q_mass of every configured
+        point source, m^3/s); times=() removes it."""
+        k = {"rain": 0, "sources": 1}[which]
+        t = np.ascontiguousarray(times, np.float64)
+        v = np.ascontiguousarray(values, np.float64)
+        if t.shape != v.shape or t.ndim != 1:
+            raise ValueError("times and values must be 1-D arrays of one length")
+        _check(lib().hf2d_set_forcing_series(self._h, k, int(t.size), _p(t) if t.size else None,
+                                             _p(v) if v.size else None, float(time_div), float(scale_mul),
+                                             float(scale_div)), self._h)
+
+    def set_gauges(self, cells, capacity: int):
+        """Record (t, h at each (row, col) of `cells`) after every driver
+        step, into a device buffer of `capacity` rows; the current state is
+        the first row."""
+        rows = np.ascontiguousarray([c[0] for c in cells], np.int64)
+        cols = np.ascontiguousarray([c[1] for c in cells], np.int64)
+        self._n_gauges = len(rows)
+        i64 = C.POINTER(C.c_int64)
+        _check(lib().hf2d_set_gauges(self._h, len(rows), rows.ctypes.data_as(i64), cols.ctypes.data_as(i64),
+                                     int(capacity)), self._h)
+
+    def read_gauges(self):
+        """(times (n_rows,), depths (n_rows, n_gauges)) recorded so far."""
+        n = C.c_int64()
+        _check(lib().hf2d_read_gauges(self._h, 0, None, C.byref(n)), self._h)
+        buf = np.zeros((n.value, self._n_gauges + 1))
+        if n.value:
+            _check(lib().hf2d_read_gauges(self._h, n.value, _p(buf), C.byref(n)), self._h)
+        return buf[:, 0].copy(), buf[:, 1:].copy()
 
     def total_volume(self) -> float:
         """sum(h) dx dy of the current state (FP64, compensated)."""

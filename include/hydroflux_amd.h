@@ -147,6 +147,33 @@ int hf2d_configure(hf2d_sim* sim, const hf2d_config* config, const hf2d_point_so
  * the dt taken is returned. */
 int hf2d_step(hf2d_sim* sim, double dt_cap, double* dt_taken);
 
+/* Time-varying forcing of the driver, the operational loops' pattern
+ * (uk_ea_test4_official.rs: apply_point_sources(Q(t_new)) after each
+ * step; uk_ea_test8a_official.rs:206-217: apply_rain(rate(t_new)) then the
+ * point sources): a piecewise-linear series evaluated on the device at
+ * t_new = t + dt of each step as
+ *     value = (interp(t_new / time_div) * scale_mul) / scale_div
+ * with interp as uk_ea_common/mod.rs (clamped at both ends). which =
+ * HF2D_SERIES_RAIN replaces the configured rain rate [m/s];
+ * HF2D_SERIES_SOURCES replaces the rate q_mass [m^3/s] of every configured
+ * point source. n = 0 removes the series. Applies to hf2d_step, run_until,
+ * run_steps and enqueue_steps. */
+enum hf2d_series_kind { HF2D_SERIES_RAIN = 0, HF2D_SERIES_SOURCES = 1 };
+int hf2d_set_forcing_series(hf2d_sim* sim, int32_t which, int32_t n, const double* t, const double* v,
+                            double time_div, double scale_mul, double scale_div);
+
+/* Gauges: the operational loops record h at control cells after every
+ * step (uk_ea_test4_official.rs, sim_depths). Here the driver appends a row
+ * (t, h at each of the n cells) to a device buffer of `capacity` rows after
+ * every step; the current (t, h) is the first row. Rows past capacity are
+ * dropped (hf2d_read_gauges reports how many were recorded). n = 0 removes
+ * the gauges. */
+int hf2d_set_gauges(hf2d_sim* sim, int32_t n, const int64_t* rows, const int64_t* cols, int64_t capacity);
+/* Copy up to max_rows rows of (1 + n) doubles into out (nullable);
+ * *n_rows = rows recorded so far (may exceed capacity if rows were dropped:
+ * it is then capped at capacity). */
+int hf2d_read_gauges(hf2d_sim* sim, int64_t max_rows, double* out, int64_t* n_rows);
+
 /* Simulation::run_until (sim.rs:231-246). The clock, dt and the step budget
  * stay on the device; the host only polls for completion. */
 int hf2d_run_until(hf2d_sim* sim, double t_target);

@@ -35,6 +35,7 @@
 // the dry clamp. `pow_mode` selects glibc pow (what the Rust reference calls)
 // or the correctly rounded routine in hf_pow.h that the GPU kernels use.
 #pragma once
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -869,12 +870,46 @@ struct Config {
   double rain_rate = 0.0;
 };
 
+// A piecewise-linear forcing series of the reference's operational loops:
+// interp of solver-2d/examples/uk_ea_common/mod.rs (clamped at both ends,
+// partition_point, frac = (t - t0) / (t1 - t0), v0 + frac * (v1 - v0)) on
+// t / time_div, then (v * scale_mul) / scale_div as uk_ea_test4_official.rs
+// (discharge_at, / inlet cells) and uk_ea_test8a_official.rs (interp_min,
+// rain_rate_m_s, point_discharge_m3_s) scale it.
+struct Series {
+  std::vector<double> t, v;
+  double time_div = 1.0, scale_mul = 1.0, scale_div = 1.0;
+  bool on() const { return !t.empty(); }
+  double at(double t_s) const {
+    const double x = t_s / time_div;
+    double val;
+    if (x <= t.front()) {
+      val = v.front();
+    } else if (x >= t.back()) {
+      val = v.back();
+    } else {
+      const size_t idx = (size_t)(std::partition_point(t.begin(), t.end(), [&](double ti) { return ti <= x; }) -
+                                  t.begin());
+      const double t0 = t[idx - 1], t1 = t[idx];
+      const double v0 = v[idx - 1], v1 = v[idx];
+      const double frac = (x - t0) / (t1 - t0);
+      val = v0 + frac * (v1 - v0);
+    }
+    return (val * scale_mul) / scale_div;
+  }
+};
+
 template <class T> struct Sim {
   Mesh<T> mesh;
   std::vector<Cell<T>> st;
   Bcs bcs;
   Config cfg;
   std::vector<PointSource> srcs;
+  // the operational loops' time-varying forcing at t_new (rain rate, and
+  // the rate of every point source) and their control-point records
+  Series rain_series, src_series;
+  std::vector<int64_t> gauges;
+  std::vector<double> gauge_rows;  // rows of (t, h at each gauge)
   Workspace<T> ws;
   double t = 0.0;
   int64_t steps = 0;
@@ -905,12 +940,27 @@ template <class T> struct Sim {
     else
       ssprk2_step_with(st, mesh, bcs, dt, ws);
     if (cfg.friction) manning_friction_step(st, mesh, dt, cfg.friction_dry_tol);
-    apply_rain(st, cfg.rain_rate, dt);
-    apply_point_sources(st, mesh.nc, srcs, dt, mesh.dx, mesh.dy);
+    const double t_new = t + dt;  // sim.time() after Simulation::step
+    apply_rain(st, rain_series.on() ? rain_series.at(t_new) : cfg.rain_rate, dt);
+    if (src_series.on()) {
+      std::vector<PointSource> now = srcs;
+      const double q = src_series.at(t_new);
+      for (PointSource& p : now) p.q_mass = q;
+      apply_point_sources(st, mesh.nc, now, dt, mesh.dx, mesh.dy);
+    } else {
+      apply_point_sources(st, mesh.nc, srcs, dt, mesh.dx, mesh.dy);
+    }
     t += dt;
     steps += 1;
     if (dt_taken) *dt_taken = dt;
+    record_gauges();
     return ST_OK;
+  }
+
+  void record_gauges() {
+    if (gauges.empty()) return;
+    gauge_rows.push_back(t);
+    for (int64_t c : gauges) gauge_rows.push_back(value(st[(size_t)c].h));
   }
 
   // sim.rs:231-246

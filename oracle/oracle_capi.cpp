@@ -21,6 +21,13 @@ struct Handle {
   Sim<Dual> d;
 };
 
+template <class T> void set_gauges_t(Sim<T>& s, int n, const int64_t* rows, const int64_t* cols) {
+  s.gauges.clear();
+  s.gauge_rows.clear();
+  for (int k = 0; k < n; ++k) s.gauges.push_back(rows[k] * s.mesh.nc + cols[k]);
+  s.record_gauges();
+}
+
 Bc make_bc(int kind, double p) {
   Bc b;
   b.kind = kind;
@@ -224,6 +231,39 @@ int hfo_run_until(void* p, double t_target) {
 int hfo_run_steps(void* p, int64_t n) {
   Handle* h = static_cast<Handle*>(p);
   return h->dual ? h->d.run_steps(n) : h->f.run_steps(n);
+}
+
+// Driver forcing series (which: 0 rain, 1 point sources; n = 0 removes).
+void hfo_set_series(void* p, int which, int n, const double* t, const double* v, double time_div,
+                    double scale_mul, double scale_div) {
+  Handle* h = static_cast<Handle*>(p);
+  Series s;
+  s.t.assign(t, t + n);
+  s.v.assign(v, v + n);
+  s.time_div = time_div;
+  s.scale_mul = scale_mul;
+  s.scale_div = scale_div;
+  if (h->dual)
+    (which == 0 ? h->d.rain_series : h->d.src_series) = s;
+  else
+    (which == 0 ? h->f.rain_series : h->f.src_series) = s;
+}
+
+// Gauges: record (t, h) at the cells after every driver step; the current
+// state is the first row.
+void hfo_set_gauges(void* p, int n, const int64_t* rows, const int64_t* cols) {
+  Handle* h = static_cast<Handle*>(p);
+  if (h->dual)
+    set_gauges_t(h->d, n, rows, cols);
+  else
+    set_gauges_t(h->f, n, rows, cols);
+}
+int64_t hfo_gauge_values(void* p, double* out, int64_t max_values) {
+  Handle* h = static_cast<Handle*>(p);
+  const std::vector<double>& r = h->dual ? h->d.gauge_rows : h->f.gauge_rows;
+  const int64_t n = (int64_t)r.size();
+  if (out) std::memcpy(out, r.data(), (size_t)(n < max_values ? n : max_values) * sizeof(double));
+  return n;
 }
 
 void hfo_time(void* p, double* t, int64_t* steps) {

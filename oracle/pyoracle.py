@@ -54,6 +54,10 @@ def lib():
         L.hfo_run_steps.restype = C.c_int
         L.hfo_run_steps.argtypes = [C.c_void_p, C.c_int64]
         L.hfo_time.argtypes = [C.c_void_p, _D, _I64]
+        L.hfo_set_series.argtypes = [C.c_void_p, C.c_int, C.c_int, _D, _D, C.c_double, C.c_double, C.c_double]
+        L.hfo_set_gauges.argtypes = [C.c_void_p, C.c_int, _I64, _I64]
+        L.hfo_gauge_values.restype = C.c_int64
+        L.hfo_gauge_values.argtypes = [C.c_void_p, _D, C.c_int64]
         L.hfo_set_time.argtypes = [C.c_void_p, C.c_double, C.c_int64]
         for name in ("hfo_pow_4_3", "hfo_pow_1_3m"):
             getattr(L, name).restype = C.c_double
@@ -167,6 +171,25 @@ class OracleSim:
 
     def run_until(self, t_target):
         return lib().hfo_run_until(self._h, float(t_target))
+
+    def set_forcing_series(self, which, times, values, time_div=1.0, scale_mul=1.0, scale_div=1.0):
+        t = np.ascontiguousarray(times, np.float64)
+        v = np.ascontiguousarray(values, np.float64)
+        lib().hfo_set_series(self._h, {"rain": 0, "sources": 1}[which], int(t.size), _p(t), _p(v),
+                             float(time_div), float(scale_mul), float(scale_div))
+
+    def set_gauges(self, cells, capacity=None):
+        rows = np.ascontiguousarray([c[0] for c in cells], np.int64)
+        cols = np.ascontiguousarray([c[1] for c in cells], np.int64)
+        self._n_gauges = len(rows)
+        lib().hfo_set_gauges(self._h, len(rows), rows.ctypes.data_as(_I64), cols.ctypes.data_as(_I64))
+
+    def read_gauges(self):
+        n = lib().hfo_gauge_values(self._h, None, 0)
+        buf = np.zeros(n)
+        lib().hfo_gauge_values(self._h, _p(buf), n)
+        buf = buf.reshape(-1, self._n_gauges + 1)
+        return buf[:, 0].copy(), buf[:, 1:].copy()
 
     def run_steps(self, n):
         return lib().hfo_run_steps(self._h, int(n))
