@@ -62,3 +62,51 @@ def ea_test4_grid(dem_path: str, kind: str = "f64", gauge_capacity: int = 60_000
         cells.append(c)
     g.set_gauges(cells, gauge_capacity)
     return g, cells
+
+
+# --------------------------------------------------------------- Test 8A
+# UK EA Test 8A (report SC120002 section 4.9; Glasgow urban, official 2 m
+# geometry) as solver-2d/examples/uk_ea_test8a_official.rs runs it: 481 x 199
+# DEM with buildings as raised bed, Manning from the official raster (0.02
+# roads, 0.05 elsewhere), all edges walls, uniform rain 400 mm/h over
+# t in [1, 4] min, a point inflow QVAR(t) * cellsize at (264894, 664750),
+# CFL 0.4, max_dt 10 s, 18000 s; rain(t_new) then the inflow after each step.
+T8_MAX_DT = 10.0
+T8_POINT_XY = (264_894.0, 664_750.0)
+T8_RAIN_MIN = ((0.0, 0.0), (0.9999, 0.0), (1.0, 400.0), (4.0, 400.0), (4.0001, 0.0), (300.0, 0.0))
+T8_POINT_MIN = ((0.0, 0.0), (20.0, 0.0), (23.0, 0.03125), (25.0, 0.125), (27.0, 0.3125), (29.0, 0.65625),
+                (31.0, 1.21875), (33.0, 1.78125), (35.0, 2.3125), (37.0, 2.5), (39.0, 2.5), (41.0, 2.3125),
+                (43.0, 1.78125), (45.0, 1.21875), (47.0, 0.65625), (49.0, 0.3125), (51.0, 0.125),
+                (53.0, 0.03125), (55.0, 0.0), (57.0, 0.0), (75.0, 0.0), (120.0, 0.0), (300.0, 0.0))
+T8_CONTROL_POINTS = ((264_680.0, 664_582.0), (264_536.0, 664_668.0), (264_354.0, 664_490.0),
+                     (264_200.0, 664_556.0), (264_332.0, 664_564.0), (264_572.0, 664_556.0),
+                     (264_708.0, 664_702.0), (264_306.0, 664_650.0), (264_220.0, 664_614.0))
+
+
+def ea_test8a_grid(dem_path: str, manning_path: str, kind: str = "f64",
+                   gauge_capacity: int = 150_000) -> Tuple[Grid2D, List[Tuple[int, int]]]:
+    """A Grid2D set up for Test 8A at t = 0 (gauges armed), and the control
+    cells (row, col)."""
+    bed, hdr = read_ascii_grid(dem_path)
+    man, mhdr = read_ascii_grid(manning_path)
+    if (mhdr.ncols, mhdr.nrows) != (hdr.ncols, hdr.nrows):
+        raise ValueError("DEM and Manning grids must share geometry")
+    if not np.all(np.isfinite(bed)):
+        raise ValueError("Test 8A DEM has no NODATA cells")
+    g = Grid2D(hdr.nrows, hdr.ncols, hdr.cellsize, hdr.cellsize, kind=kind)
+    g.set_mesh(bed, man)
+    g.set_state(np.zeros((hdr.nrows, hdr.ncols, 3)))
+    g.set_boundaries([WALL, WALL, WALL, WALL])
+    cell = hdr.cell_at(*T8_POINT_XY)
+    if cell is None:
+        raise ValueError("point inflow falls outside the grid")
+    g.configure(cfl=CFL, max_dt=T8_MAX_DT, sources=[(cell[0], cell[1], 0.0)])
+    t, v = zip(*T8_RAIN_MIN)
+    g.set_forcing_series("rain", t, v, time_div=60.0, scale_mul=1e-3, scale_div=3600.0)
+    t, v = zip(*T8_POINT_MIN)
+    g.set_forcing_series("sources", t, v, time_div=60.0, scale_mul=hdr.cellsize, scale_div=1.0)
+    cells = [hdr.cell_at(x, y) for x, y in T8_CONTROL_POINTS]
+    if any(c is None for c in cells):
+        raise ValueError("a control point falls outside the grid")
+    g.set_gauges(cells, gauge_capacity)
+    return g, cells

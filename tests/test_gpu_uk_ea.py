@@ -130,3 +130,73 @@ def test_uk_ea_test4_full_run_reproduces_published_comparison():
             assert abs(cmp.peak_bias - bias) <= 0.5e-4 + 1e-12, (name, p, cmp.peak_bias)
             assert cmp.sim_arrival is not None and cmp.reference_arrival is not None
             assert abs((cmp.sim_arrival - cmp.reference_arrival) - arrival) <= 0.05 + 1e-9, (name, p)
+
+
+# The reference's run of Test 8A with its current step (after the
+# source-term change of docs/bug-report-2026-07-boundary-slope-instability.md
+# section 10), papers/01_review/ROADMAP_REVISION_EMS.md:222-229: 141,573
+# steps, depths at control points 1, 2, 4, 7 (peak) and 3 (peak / final) to
+# 4 decimals. (The earlier table in test8a_glasgow/results_hydroflux.md,
+# 142,074 steps, predates that change.)
+T8A = os.path.join(os.path.dirname(__file__), "golden", "uk_ea", "test8a")
+POSTFIX_8A_STEPS = 141_573
+POSTFIX_8A_PEAKS = {0: 0.5534, 1: 0.2477, 2: 0.7340, 3: 0.1629, 6: 0.2427}
+POSTFIX_8A_FINAL_PT3 = 0.7339
+
+
+def _oracle_test8a(t_end, pow_mode):
+    from hydroflux_amd import uk_ea
+    from hydroflux_amd.ascii_grid import read_ascii_grid
+
+    bed, hdr = read_ascii_grid(os.path.join(T8A, "ea8-2m.dem.gz"))
+    man, _ = read_ascii_grid(os.path.join(T8A, "ea8-2m.n.gz"))
+    set_pow_mode(pow_mode)
+    o = OracleSim(hdr.nrows, hdr.ncols, hdr.cellsize, hdr.cellsize)
+    o.set_mesh(bed, man)
+    o.set_state(np.zeros((hdr.nrows, hdr.ncols, 3)))
+    o.set_boundaries([uk_ea.WALL] * 4)
+    cell = hdr.cell_at(*uk_ea.T8_POINT_XY)
+    o.configure(cfl=uk_ea.CFL, max_dt=uk_ea.T8_MAX_DT, sources=[(cell[0], cell[1], 0.0)])
+    t, v = zip(*uk_ea.T8_RAIN_MIN)
+    o.set_forcing_series("rain", t, v, 60.0, 1e-3, 3600.0)
+    t, v = zip(*uk_ea.T8_POINT_MIN)
+    o.set_forcing_series("sources", t, v, 60.0, hdr.cellsize, 1.0)
+    o.set_gauges([hdr.cell_at(x, y) for x, y in uk_ea.T8_CONTROL_POINTS])
+    assert o.run_until(t_end) == 0
+    return o
+
+
+def test_uk_ea_test8a_first_300s_bitwise_with_oracle():
+    """Urban topography, spatially varying Manning, the rain pulse and the
+    start of the inflow, bit for bit. (Against glibc's pow instead of the
+    kernels' correctly rounded one, single cells already differ by
+    centimetres after 300 s here: one-ulp friction differences flip cells
+    across the H_VEL / H_DRY switches in the thin roof and street films, so
+    only identical arithmetic gives a meaningful comparison.)"""
+    _say("UK EA test 8A, 0-300 s, gpu vs oracle")
+    from hydroflux_amd import uk_ea
+
+    g, _ = uk_ea.ea_test8a_grid(os.path.join(T8A, "ea8-2m.dem.gz"), os.path.join(T8A, "ea8-2m.n.gz"))
+    g.run_until(300.0)
+    o = _oracle_test8a(300.0, 1)
+    assert g.time() == o.time()
+    assert np.array_equal(g.get_state(), o.get_state())
+    assert np.array_equal(g.read_gauges()[1], o.read_gauges()[1])
+
+
+def test_uk_ea_test8a_full_run_reproduces_reference_run():
+    _say("UK EA test 8A, full 18000 s")
+    from hydroflux_amd import uk_ea
+
+    g, _ = uk_ea.ea_test8a_grid(os.path.join(T8A, "ea8-2m.dem.gz"), os.path.join(T8A, "ea8-2m.n.gz"))
+    g.run_until(uk_ea.SIM_TIME)
+    t, steps = g.time()
+    times, depths = g.read_gauges()
+    peaks = np.maximum(depths.max(axis=0), 0.0)
+    print(f"[test8a] steps {steps}, peaks {peaks.tolist()}, final pt3 {depths[-1, 2]!r}", flush=True)
+    assert t == uk_ea.SIM_TIME and steps == POSTFIX_8A_STEPS
+    assert len(times) == steps + 1
+    # one unit of the printed 4th decimal
+    for p, ref in POSTFIX_8A_PEAKS.items():
+        assert abs(peaks[p] - ref) <= 1e-4, (p, peaks[p], ref)
+    assert abs(depths[-1, 2] - POSTFIX_8A_FINAL_PT3) <= 1e-4
