@@ -32,6 +32,10 @@ Beside it in the same JSON line (unless --quick):
                   Dual tangents on one land-cover class's Manning n, at a
                   sweep of arrival rates: p50/p99 latency-to-completion and
                   aggregate cell-updates/s;
+  dry_domain      the compacted active-tile list on the 4096^2 workload
+                  without rain (inflow only), on and off;
+  uk_ea           UK EA Tests 4 and 8A on their official geometries, full
+                  18000 s runs, wall time;
   cpu_baseline    the C++ restatement of the reference step, 1 thread, on a
                   bounded 1024^2 sample.
 
@@ -260,6 +264,39 @@ def dry_domain(args):
     return out
 
 
+def uk_ea(args):
+    """The UK EA official benchmarks end to end on the GPU (device-side
+    hydrograph / hyetograph series and gauges; 18000 simulated s each):
+    Test 4 (400 x 200, 5 m) and Test 8A (Glasgow, 199 x 481, 2 m). The
+    reference's recorded CPU wall times of the same runs, on its authors'
+    machine (papers/01_review/ROADMAP_REVISION_EMS.md:211-229), are given for
+    context only."""
+    from hydroflux_amd import uk_ea as U
+
+    g4 = os.path.join(ROOT, "tests", "golden", "uk_ea", "test4")
+    g8 = os.path.join(ROOT, "tests", "golden", "uk_ea", "test8a")
+    out = {}
+    for name, make, ref_s in (
+        ("test4", lambda: U.ea_test4_grid(os.path.join(g4, "ea4-5m.dem.gz")), 1136.0),
+        ("test8a", lambda: U.ea_test8a_grid(os.path.join(g8, "ea8-2m.dem.gz"), os.path.join(g8, "ea8-2m.n.gz")),
+         5289.0),
+    ):
+        g, _ = make()
+        cells = g.n_rows * g.n_cols
+        g.synchronize()
+        t0 = time.perf_counter()
+        g.run_until(U.SIM_TIME)
+        g.synchronize()
+        wall = time.perf_counter() - t0
+        steps = g.time()[1]
+        out[name] = {"grid": [g.n_rows, g.n_cols], "steps": steps, "wall_s": wall,
+                     "cell_updates_per_s": cells * steps / wall, "us_per_step": wall / steps * 1e6,
+                     "reference_cpu_wall_s": ref_s}
+        g.close()
+        log(f"[bench] UK EA {name}: {steps} steps in {wall:.2f} s ({wall / steps * 1e6:.1f} us/step)")
+    return out
+
+
 def batch_bench(args):
     from hydroflux_amd import Grid2D
     from hydroflux_amd.batch import Batch, ScenarioSpec
@@ -356,8 +393,8 @@ def main():
     ap.add_argument("--size", type=int, default=4096)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="headline FP64 line only (and cpu_baseline)")
-    ap.add_argument("--sections", default="precision,c1,dry,batch,serving",
-                    help="comma list of extra sections: precision, c1, dry, batch, serving")
+    ap.add_argument("--sections", default="precision,c1,dry,ukea,batch,serving",
+                    help="comma list of extra sections: precision, c1, dry, ukea, batch, serving")
     ap.add_argument("--dry-warmup", type=int, default=500)
     ap.add_argument("--acc-steps", type=int, default=2000, help="FP64 steps for the FP32 accuracy check (0: skip)")
     ap.add_argument("--serve-requests", type=int, default=64)
@@ -427,6 +464,8 @@ def main():
         out["c1_1024"] = c1_1024(args)
     if "dry" in sections:
         out["dry_domain"] = dry_domain(args)
+    if "ukea" in sections:
+        out["uk_ea"] = uk_ea(args)
     if "batch" in sections or "serving" in sections:
         out["batch"], mix = batch_bench(args)
         if "serving" in sections:

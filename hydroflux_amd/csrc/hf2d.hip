@@ -288,8 +288,8 @@ template <class T> int enqueue_step(hf2d_sim* s) {
   b.rain_rate = s->cfg.rain_rate;
   b.n_src = s->n_src;
   b.wave_speed = 1;
-  a.series = b.series = s->series_dev;
-  b.use_series = 1;
+  const bool series = s->series_host[0].n > 0 || s->series_host[1].n > 0;
+  b.use_series = series ? 1 : 0;
   if (list_usable(s)) {
     int rc = enqueue_compact<T>(s);
     if (rc) return rc;
@@ -303,6 +303,10 @@ template <class T> int enqueue_step(hf2d_sim* s) {
   if (s->cfg.integrator == HF2D_SSPRK2) {
     int rc = launch_stage<T>(s, a, hf::MODE_STAGE, 0);
     if (rc) return rc;
+    if (series) {
+      hipLaunchKernelGGL(series_kernel, dim3(1), dim3(1), 0, s->stream, s->ctrl, s->series_dev);
+      HIPCHK(s, hipGetLastError());
+    }
     rc = launch_stage<T>(s, b, hf::MODE_FINAL_RK2, 1);
     if (rc) return rc;
   } else {
@@ -310,6 +314,10 @@ template <class T> int enqueue_step(hf2d_sim* s) {
     // tail copies it back with forcing and the CFL maxima.
     int rc = launch_stage<T>(s, a, hf::MODE_STAGE, 0);
     if (rc) return rc;
+    if (series) {
+      hipLaunchKernelGGL(series_kernel, dim3(1), dim3(1), 0, s->stream, s->ctrl, s->series_dev);
+      HIPCHK(s, hipGetLastError());
+    }
     StageArgs<T> t = b;
     t.ih = fld<T>(s->st[oth][0], s->tg[oth][0]);
     t.ihu = fld<T>(s->st[oth][1], s->tg[oth][1]);

@@ -181,6 +181,19 @@ __global__ __launch_bounds__(256) void tile_wet_init_kernel(Fld<T> h, int64_t nr
   }
 }
 
+// The driver's forcing series at the clock after the step about to finish
+// (t + dt, dt published by the first stage), into the control block for
+// the final stage's forcing. One thread.
+__global__ void series_kernel(Ctrl* c, const SeriesDesc* s) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  if (c->status != 0 || c->done != 0) return;
+  const double t_new = hf::clock_after(c, c->dt);
+  for (int k = 0; k < 2; ++k) {
+    c->series_on[k] = s[k].n > 0 ? 1 : 0;
+    if (s[k].n > 0) c->series_now[k] = hf::series_at(s[k], t_new);
+  }
+}
+
 // Gauges: append (t, h at each gauge cell) as row *count of buf (rows of
 // n + 1 doubles) after a step that advanced the clock, or unconditionally
 // with force (the initial row). Rows past cap are dropped.
@@ -211,11 +224,12 @@ __global__ __launch_bounds__(256) void tail_kernel(StageArgs<T> a, int slot_mode
   using St = hf::Stage<T, 32, 16>;
   double wsx = 0.0, wsy = 0.0;
   const int64_t n = a.nr * a.nc;
+  const typename St::ForcingNow fnow = St::forcing_now(a, 0, 0, a.nr, a.nc);
   for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < n;
        idx += (int64_t)gridDim.x * blockDim.x) {
     const int64_t gi = idx / a.nc, gj = idx % a.nc;
     T h = a.ih.ld(idx), hu = a.ihu.ld(idx), hv = a.ihv.ld(idx);
-    St::forcing(a, gi, gj, dt, &h, &hu, &hv);
+    St::forcing(a, fnow, gi, gj, dt, &h, &hu, &hv);
     a.oh.st(idx, h);
     a.ohu.st(idx, hu);
     a.ohv.st(idx, hv);

@@ -175,10 +175,8 @@ template <class T> struct StageArgs {
   const int* tile_list;
   const int* n_work;
   unsigned char* tile_wet;
-  // Driver forcing series ([0] rain, [1] point sources; device memory,
-  // nullable): the first stage evaluates them at t + dt into the control
-  // block, and forcing() uses those values when use_series is set.
-  const SeriesDesc* series;
+  // forcing() takes the rain rate and point-source rate from the control
+  // block's series values (series_kernel, at t + dt) when use_series is set.
   int use_series;
   // FP32 precision form: bed = datum[block] (FP64, one per 2^datum_log2
   // square block of cells) + zdev (FP32 deviation); `z` is unused then.
@@ -905,11 +903,40 @@ template <class T, int TW, int TH> struct Stage {
     }
   }
 
+  // The step's rain rate and point-source rate (the driver series'
+  // values, or the configured ones), and whether a point source lies in
+  // rows [gi0, gi0 + th) x columns [gj0, gj0 + tw). Read once per thread,
+  // before the launch stores anything, so the per-cell tail reloads none
+  // of it.
+  struct ForcingNow {
+    double rain;
+    double q;
+    int q_series;
+    int src_here;
+  };
+  static HF_HD ForcingNow forcing_now(const StageArgs<T>& a, int64_t gi0, int64_t gj0, int64_t th, int64_t tw) {
+    ForcingNow f{a.rain_rate, 0.0, 0, 0};
+    if (a.use_series) {
+      const Ctrl* c = a.ctrl;
+      if (c->series_on[0]) f.rain = c->series_now[0];
+      if (c->series_on[1]) {
+        f.q = c->series_now[1];
+        f.q_series = 1;
+      }
+    }
+    for (int k = 0; k < a.n_src; ++k) {
+      const int64_t r = a.srcs[k].row, cc = a.srcs[k].col;
+      if (r >= gi0 && r < gi0 + th && cc >= gj0 && cc < gj0 + tw) f.src_here = 1;
+    }
+    return f;
+  }
+
   // Cell-local operator-split tail in the reference callers' order
   // (sim.rs:224; uk_ea_test8a_official.rs:206-217): Manning friction
   // (source.rs:130-165), rain (source.rs:88-101), point sources
   // (source.rs:52-69).
-  static HF_HD void forcing(const StageArgs<T>& a, int64_t gi, int64_t gj, double dt, T* h, T* hu, T* hv) {
+  static HF_HD void forcing(const StageArgs<T>& a, const ForcingNow& fn, int64_t gi, int64_t gj, double dt, T* h,
+                            T* hu, T* hv) {
     const T Z = cst<T>(0.0);
     if (a.friction && val(*h) > R(a.friction_dry_tol)) {
       T n = a.man.ld(gi * a.nc + gj);
@@ -921,9 +948,7 @@ template <class T, int TW, int TH> struct Stage {
         *hv = div(*hv, factor);
       }
     }
-    const bool rain_series = a.use_series && a.ctrl->series_on[0];
-    const bool src_series = a.use_series && a.ctrl->series_on[1];
-    const double rain = rain_series ? a.ctrl->series_now[0] : a.rain_rate;
+    const double rain = fn.rain;
     if (rain != 0.0) {  // source.rs:88-101
       const R dh = R(rain * dt);
       T nh = *h + dh;
@@ -935,11 +960,11 @@ template <class T, int TW, int TH> struct Stage {
         *h = nh;
       }
     }
-    if (a.n_src > 0) {
+    if (fn.src_here) {
       const double cell_area = a.dx * a.dy;
       for (int k = 0; k < a.n_src; ++k) {
         if (a.srcs[k].row != gi || a.srcs[k].col != gj) continue;
-        const double q = src_series ? a.ctrl->series_now[1] : a.srcs[k].q;
+        const double q = fn.q_series ? fn.q : a.srcs[k].q;
         const R dh = R(q * dt / cell_area);
         *h = rmax(*h + dh, Z);
         if (val(*h) == R(0)) {
@@ -959,6 +984,8 @@ template <class T, int TW, int TH> struct Stage {
                                  double dt, double* wsx, double* wsy, bool* wet) {
     const T Z = cst<T>(0.0);
     const R dt_dx = R(dt / a.dx), dt_dy = R(dt / a.dy), rdt = R(dt);
+    ForcingNow fnow{};
+    if (MODE == MODE_FINAL_RK2) fnow = forcing_now(a, gi0, gj0, TH, TW);
     for (int k = tid; k < TW * TH; k += nt) {
       const int li = k / TW, lj = k % TW;
       const int64_t gi = gi0 + li, gj = gj0 + lj;
@@ -1066,7 +1093,7 @@ template <class T, int TW, int TH> struct Stage {
           nhu = Z;
           nhv = Z;
         }
-        forcing(a, gi, gj, dt, &nh, &nhu, &nhv);
+        forcing(a, fnow, gi, gj, dt, &nh, &nhu, &nhv);
         if (!(val(nh) <= R(H_DRY))) *wet = true;
         if (a.wave_speed) {
           T zn4[4];
@@ -1179,13 +1206,6 @@ HF_HD bool stage_prologue(const StageArgs<T>& a, bool lead, double* dt_out) {
     c->dt = dt;
     c->ws[c->parity ^ 1][0] = 0ULL;
     c->ws[c->parity ^ 1][1] = 0ULL;
-    if (a.series) {
-      const double t_new = clock_after(c, dt);
-      for (int k = 0; k < 2; ++k) {
-        c->series_on[k] = a.series[k].n > 0 ? 1 : 0;
-        if (a.series[k].n > 0) c->series_now[k] = series_at(a.series[k], t_new);
-      }
-    }
   }
   *dt_out = dt;
   return true;
