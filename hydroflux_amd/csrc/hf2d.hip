@@ -58,6 +58,12 @@ struct hf2d_sim {
   long long* gauge_count = nullptr;
   int n_gauges = 0;
   long long gauge_cap = 0;
+  // GRAPH_STEPS device-clocked steps captured once and replayed; dropped
+  // whenever a launch argument baked into it changes.
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  int graph_cur = -1;
+  bool graph_list = false;
   std::string err;
   // kernel timing
   bool timing = false;
@@ -68,6 +74,15 @@ struct hf2d_sim {
 };
 
 namespace {
+
+constexpr int64_t GRAPH_STEPS = 16;
+
+void drop_graph(hf2d_sim* s) {
+  if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
+  if (s->graph) (void)hipGraphDestroy(s->graph);
+  s->gexec = nullptr;
+  s->graph = nullptr;
+}
 
 int fail(hf2d_sim* s, int code, const std::string& msg) {
   if (s) s->err = msg;
@@ -251,15 +266,16 @@ template <class T> int host_dt_step(hf2d_sim* s, double dt, bool rk2) {
   }
   s->ws_valid = false;
   s->list_valid = false;
+  drop_graph(s);
   return HF2D_OK;
 }
 
-// Rebuild the active-tile list for the next step (after seeding the wet
-// flags from the state if it was set by other means than the driver).
-template <class T> int enqueue_compact(hf2d_sim* s) {
+// Seed the active-tile list's wet flags from the current state (after the
+// state was set by other means than the driver).
+template <class T> int enqueue_list_init(hf2d_sim* s) {
   const int64_t nt = (int64_t)s->tiles_x * s->tiles_y;
   const unsigned g = (unsigned)((nt + 255) / 256 < 4096 ? (nt + 255) / 256 : 4096);
-  if (!s->list_valid) {
+  {
     hipLaunchKernelGGL(fill_bytes_kernel, dim3(g), dim3(256), 0, s->stream, s->tile_wet, nt, (unsigned char)0);
     hipLaunchKernelGGL(fill_bytes_kernel, dim3(g), dim3(256), 0, s->stream, s->tile_run, nt, (unsigned char)0);
     hipLaunchKernelGGL((tile_wet_init_kernel<T>), dim3(tail_grid(s)), dim3(256), 0, s->stream,
@@ -267,6 +283,16 @@ template <class T> int enqueue_compact(hf2d_sim* s) {
                        s->tiles_x, s->tile_wet);
     HIPCHK(s, hipGetLastError());
     s->list_valid = true;
+  }
+  return HF2D_OK;
+}
+
+// Rebuild the active-tile list for the next step.
+template <class T> int enqueue_compact(hf2d_sim* s) {
+  const int64_t nt = (int64_t)s->tiles_x * s->tiles_y;
+  if (!s->list_valid) {
+    int rc = enqueue_list_init<T>(s);
+    if (rc) return rc;
   }
   hipLaunchKernelGGL(compact_kernel, dim3(1), dim3(1024), 0, s->stream, s->tile_wet, s->tile_pin, s->tile_run,
                      s->tile_act, s->tiles_x, s->tiles_y, s->tile_list, s->tile_list + nt);
@@ -362,6 +388,44 @@ template <class T> int record_gauges_now(hf2d_sim* s) {
   return HF2D_OK;
 }
 
+// Enqueue n device-clocked steps: whole captured GRAPH_STEPS-step graphs
+// where possible (launch-bound small grids), the rest one by one. Kernel
+// timing records events per step, so it runs without the graph.
+template <class T> int enqueue_n(hf2d_sim* s, int64_t n) {
+  if (!s->timing && n >= GRAPH_STEPS) {
+    const bool lu = list_usable(s);
+    if (lu && !s->list_valid) {
+      int rc = enqueue_list_init<T>(s);
+      if (rc) return rc;
+    }
+    if (!s->gexec || s->graph_cur != s->cur || s->graph_list != lu) {
+      drop_graph(s);
+      HIPCHK(s, hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+      int rc = HF2D_OK;
+      for (int64_t k = 0; k < GRAPH_STEPS && rc == HF2D_OK; ++k) rc = enqueue_step<T>(s);
+      hipGraph_t g = nullptr;
+      hipError_t e = hipStreamEndCapture(s->stream, &g);
+      if (rc == HF2D_OK && e != hipSuccess) rc = fail(s, HF2D_HIP_ERROR, hipGetErrorString(e));
+      if (rc == HF2D_OK && hipGraphInstantiate(&s->gexec, g, nullptr, nullptr, 0) != hipSuccess)
+        rc = fail(s, HF2D_HIP_ERROR, "hipGraphInstantiate failed");
+      if (rc != HF2D_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        s->gexec = nullptr;
+        return rc;
+      }
+      s->graph = g;
+      s->graph_cur = s->cur;
+      s->graph_list = lu;
+    }
+    for (; n >= GRAPH_STEPS; n -= GRAPH_STEPS) HIPCHK(s, hipGraphLaunch(s->gexec, s->stream));
+  }
+  for (int64_t k = 0; k < n; ++k) {
+    int rc = enqueue_step<T>(s);
+    if (rc) return rc;
+  }
+  return HF2D_OK;
+}
+
 template <class T> int prepare_driver(hf2d_sim* s) {
   return refresh_wave_speeds<T>(s, s->cfg.cfl_with_bcs != 0);
 }
@@ -408,10 +472,8 @@ template <class T> int run_driver(hf2d_sim* s, bool use_target, double t_target,
   if (rc) return rc;
   s->ws_valid = false;  // the slots now follow the driver
   if (!use_target) {
-    for (int64_t k = 0; k < n_steps; ++k) {
-      rc = enqueue_step<T>(s);
-      if (rc) return rc;
-    }
+    rc = enqueue_n<T>(s, n_steps);
+    if (rc) return rc;
     if (!poll) return HF2D_OK;
     rc = read_ctrl(s, &c);
     if (rc) return rc;
@@ -424,10 +486,8 @@ template <class T> int run_driver(hf2d_sim* s, bool use_target, double t_target,
   // run_until: enqueue in chunks, poll the device clock between chunks.
   int64_t chunk = 16;
   for (;;) {
-    for (int64_t k = 0; k < chunk; ++k) {
-      rc = enqueue_step<T>(s);
-      if (rc) return rc;
-    }
+    rc = enqueue_n<T>(s, chunk);
+    if (rc) return rc;
     rc = read_ctrl(s, &c);
     if (rc) return rc;
     rc = harvest_timing(s);
@@ -470,6 +530,7 @@ template <class T> int forcing_only(hf2d_sim* s, double dt, int friction, double
   HIPCHK(s, hipGetLastError());
   s->ws_valid = false;
   s->list_valid = false;
+  drop_graph(s);
   return HF2D_OK;
 }
 
@@ -590,6 +651,7 @@ int hf2d_create(int64_t n_rows, int64_t n_cols, double dx, double dy, int32_t sc
 int hf2d_destroy(hf2d_sim* s) {
   if (!s) return HF2D_OK;
   if (s->stream) (void)hipStreamSynchronize(s->stream);
+  drop_graph(s);
   for (int b = 0; b < 2; ++b)
     for (int f = 0; f < 3; ++f) {
       if (s->st[b][f]) (void)hipFree(s->st[b][f]);
@@ -656,6 +718,7 @@ int hf2d_upload_mesh(hf2d_sim* s, const double* bed, const double* bed_tangent, 
   }
   s->ws_valid = false;
   s->list_valid = false;
+  drop_graph(s);
   return HF2D_OK;
 }
 
@@ -702,6 +765,7 @@ int hf2d_upload_state(hf2d_sim* s, const double* aos, const double* tan_aos) {
   }
   s->ws_valid = false;
   s->list_valid = false;
+  drop_graph(s);
   return HF2D_OK;
 }
 
@@ -760,6 +824,7 @@ int hf2d_set_boundaries(hf2d_sim* s, const int32_t kinds[4], const double params
   }
   s->ws_valid = false;
   s->list_valid = false;
+  drop_graph(s);
   return HF2D_OK;
 }
 
@@ -824,6 +889,7 @@ int hf2d_apply_point_sources(hf2d_sim* s, const hf2d_point_source* src, int64_t 
   s->cap_src = saved_cap;
   s->ws_valid = false;
   s->list_valid = false;
+  drop_graph(s);
   return rc;
 }
 
@@ -849,6 +915,7 @@ int hf2d_configure(hf2d_sim* s, const hf2d_config* cfg, const hf2d_point_source*
   s->cfg = *cfg;
   s->ws_valid = false;
   s->list_valid = false;
+  drop_graph(s);
   return HF2D_OK;
 }
 
@@ -882,11 +949,13 @@ int hf2d_set_forcing_series(hf2d_sim* s, int32_t which, int32_t n, const double*
   if (!s->series_dev) HIPCHK(s, hipMalloc((void**)&s->series_dev, 2 * sizeof(SeriesDesc)));
   HIPCHK(s, hipMemcpy(s->series_dev, s->series_host, 2 * sizeof(SeriesDesc), hipMemcpyHostToDevice));
   s->list_valid = false;
+  drop_graph(s);
   return HF2D_OK;
 }
 
 int hf2d_set_gauges(hf2d_sim* s, int32_t n, const int64_t* rows, const int64_t* cols, int64_t capacity) {
   if (!valid(s)) return HF2D_INVALID_ARGUMENT;
+  drop_graph(s);
   if (n < 0 || (n > 0 && (!rows || !cols)) || (n > 0 && capacity < 1))
     return fail(s, HF2D_INVALID_ARGUMENT, "invalid gauge arguments");
   std::vector<int64_t> cells((size_t)n);
@@ -996,6 +1065,7 @@ int hf2d_set_kernel_timing(hf2d_sim* s, int32_t enabled) {
   int rc = harvest_timing(s);
   if (rc) return rc;
   s->timing = enabled != 0;
+  drop_graph(s);
   s->timed_ms[0] = s->timed_ms[1] = 0.0;
   s->timed_count = 0;
   s->ev_used = 0;

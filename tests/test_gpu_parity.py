@@ -178,6 +178,40 @@ def test_active_tile_list_bitwise(kind, west):
             assert np.array_equal(a, r), "gpu != oracle"
 
 
+@pytest.mark.parametrize("rain", [0.0, 2e-4], ids=["list", "rain"])
+def test_graph_replays_interleaved_with_host_calls_bitwise(rain):
+    """The driver replays captured 16-step graphs; eager steps, host-side
+    state changes and reconfiguration between replays must not leave a
+    stale graph or a stale active-tile list."""
+    _say(f"graph replays rain={rain}")
+    c = cases.mostly_dry(90, 130, 6, west=cases.DEPTH)
+    g, o = _pair(c)
+    src = [(c.nr - 9, 11, 0.05)]
+    for s in (g, o):
+        s.configure(cfl=0.4, max_dt=0.2, rain_rate=rain, sources=src)
+    for s in (g, o):
+        s.run_steps(40)           # 2 replays + 8 eager steps
+    assert np.array_equal(g.get_state(), o.get_state())
+    for s in (g, o):
+        s.apply_rain(1e-3, 0.5)   # host-side state change between replays
+        s.run_steps(37)
+    assert g.time() == o.time() and np.array_equal(g.get_state(), o.get_state())
+    st = g.get_state()
+    st[45, 60] = (0.4, 0.1, 0.0)
+    for s in (g, o):
+        s.set_state(st)
+        s.configure(cfl=0.35, max_dt=0.1, rain_rate=rain, sources=src + [(50, 70, 0.01)])
+        s.run_until(s.time()[0] + 3.0)
+    assert g.time() == o.time() and np.array_equal(g.get_state(), o.get_state())
+    g.set_kernel_timing(True)     # eager path with events, then graphs again
+    o.run_steps(20)
+    g.run_steps(20)
+    g.set_kernel_timing(False)
+    for s in (g, o):
+        s.run_steps(48)
+    assert g.time() == o.time() and np.array_equal(g.get_state(), o.get_state())
+
+
 def test_forward_euler_driver_bitwise():
     _say("forward euler driver")
     c = cases.bumpy_wet_dry(40, 70, 5)
