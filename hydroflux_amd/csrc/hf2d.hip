@@ -24,9 +24,10 @@ struct hf2d_sim {
   int64_t nr = 0, nc = 0;
   double dx = 1.0, dy = 1.0;
   int kind = HF2D_F64;
+  int n_tan = 0;  // tangents per value: Dual 1, DualN<N> N, else 0
   hipStream_t stream = nullptr;
   void* st[2][3] = {{nullptr}};   // state buffers (double, or float for HF2D_F32)
-  void* tg[2][3] = {{nullptr}};   // tangents (Dual)
+  void* tg[2][3] = {{nullptr}};   // tangents (Dual; DualN: N per cell, [cell][k])
   void *man = nullptr, *mant = nullptr;
   double *z = nullptr, *zt = nullptr;  // bed (double / Dual kinds)
   double* datum = nullptr;             // FP32 kind: FP64 datum per 64x64 block
@@ -578,8 +579,25 @@ int upload_pins(hf2d_sim* s, const hf2d_point_source* src, int64_t k) {
 }
 }  // namespace
 
+using dual2 = hf::dualn<2>;
+using dual4 = hf::dualn<4>;
+using dual8 = hf::dualn<8>;
+
+static int tangents_of(int kind) {
+  switch (kind) {
+    case HF2D_DUAL: return 1;
+    case HF2D_DUAL2: return 2;
+    case HF2D_DUAL4: return 4;
+    case HF2D_DUAL8: return 8;
+    default: return 0;
+  }
+}
+
 #define HF_DISPATCH(s, FN, ...)                                  \
   ((s)->kind == HF2D_DUAL  ? FN<dual>(__VA_ARGS__)                \
+   : (s)->kind == HF2D_DUAL2 ? FN<dual2>(__VA_ARGS__)             \
+   : (s)->kind == HF2D_DUAL4 ? FN<dual4>(__VA_ARGS__)             \
+   : (s)->kind == HF2D_DUAL8 ? FN<dual8>(__VA_ARGS__)             \
    : (s)->kind == HF2D_F32 ? FN<float>(__VA_ARGS__)               \
                            : FN<double>(__VA_ARGS__))
 
@@ -589,7 +607,8 @@ extern "C" {
 int hf2d_create(int64_t n_rows, int64_t n_cols, double dx, double dy, int32_t scalar_kind, hf2d_sim** out) {
   if (!out) return HF2D_INVALID_ARGUMENT;
   *out = nullptr;
-  if (scalar_kind != HF2D_F64 && scalar_kind != HF2D_DUAL && scalar_kind != HF2D_F32) return HF2D_INVALID_ARGUMENT;
+  if (scalar_kind != HF2D_F64 && scalar_kind != HF2D_F32 && tangents_of(scalar_kind) == 0)
+    return HF2D_INVALID_ARGUMENT;
   if (!(dx > 0.0) || !(dy > 0.0) || n_rows <= 0 || n_cols <= 0) return HF2D_INVALID_CONFIG;
   hf2d_sim* s = new hf2d_sim();
   s->nr = n_rows;
@@ -597,9 +616,11 @@ int hf2d_create(int64_t n_rows, int64_t n_cols, double dx, double dy, int32_t sc
   s->dx = dx;
   s->dy = dy;
   s->kind = scalar_kind;
+  s->n_tan = tangents_of(scalar_kind);
   s->cfg = default_config();
   const size_t n = (size_t)(n_rows * n_cols);
   const size_t bytes = n * elem_bytes(scalar_kind);
+  const size_t tbytes = bytes * (size_t)s->n_tan;
   auto cleanup = [&](int code) {
     hf2d_destroy(s);
     return code;
@@ -611,7 +632,7 @@ int hf2d_create(int64_t n_rows, int64_t n_cols, double dx, double dy, int32_t sc
   for (int b = 0; b < 2; ++b)
     for (int f = 0; f < 3; ++f) {
       if (!zalloc(&s->st[b][f], bytes)) return cleanup(HF2D_HIP_ERROR);
-      if (scalar_kind == HF2D_DUAL && !zalloc(&s->tg[b][f], bytes)) return cleanup(HF2D_HIP_ERROR);
+      if (s->n_tan > 0 && !zalloc(&s->tg[b][f], tbytes)) return cleanup(HF2D_HIP_ERROR);
     }
   if (!zalloc(&s->man, bytes)) return cleanup(HF2D_HIP_ERROR);
   if (scalar_kind == HF2D_F32) {
@@ -622,11 +643,11 @@ int hf2d_create(int64_t n_rows, int64_t n_cols, double dx, double dy, int32_t sc
   } else {
     if (!zalloc((void**)&s->z, n * sizeof(double))) return cleanup(HF2D_HIP_ERROR);
   }
-  if (scalar_kind == HF2D_DUAL) {
-    if (!zalloc((void**)&s->zt, bytes)) return cleanup(HF2D_HIP_ERROR);
-    if (!zalloc(&s->mant, bytes)) return cleanup(HF2D_HIP_ERROR);
+  if (s->n_tan > 0) {
+    if (!zalloc((void**)&s->zt, tbytes)) return cleanup(HF2D_HIP_ERROR);
+    if (!zalloc(&s->mant, tbytes)) return cleanup(HF2D_HIP_ERROR);
   }
-  if (scalar_kind != HF2D_DUAL) {
+  if (s->n_tan == 0) {
     const int tw = scalar_kind == HF2D_F32 ? Shape<float>::TW : Shape<double>::TW;
     const int th = scalar_kind == HF2D_F32 ? Shape<float>::TH : Shape<double>::TH;
     s->tiles_x = (int)((n_cols + tw - 1) / tw);
@@ -705,16 +726,17 @@ int hf2d_upload_mesh(hf2d_sim* s, const double* bed, const double* bed_tangent, 
     HIPCHK(s, hipMemcpy(s->z, bed, bytes, hipMemcpyHostToDevice));
     HIPCHK(s, hipMemcpy(s->man, manning, bytes, hipMemcpyHostToDevice));
   }
-  if (s->kind == HF2D_DUAL) {
+  if (s->n_tan > 0) {  // tangents: n_tan per cell, [cell][k]
+    const size_t tb = bytes * (size_t)s->n_tan;
     s->has_zt = bed_tangent != nullptr;
     if (bed_tangent)
-      HIPCHK(s, hipMemcpy(s->zt, bed_tangent, bytes, hipMemcpyHostToDevice));
+      HIPCHK(s, hipMemcpy(s->zt, bed_tangent, tb, hipMemcpyHostToDevice));
     else
-      HIPCHK(s, hipMemset(s->zt, 0, bytes));
+      HIPCHK(s, hipMemset(s->zt, 0, tb));
     if (manning_tangent)
-      HIPCHK(s, hipMemcpy(s->mant, manning_tangent, bytes, hipMemcpyHostToDevice));
+      HIPCHK(s, hipMemcpy(s->mant, manning_tangent, tb, hipMemcpyHostToDevice));
     else
-      HIPCHK(s, hipMemset(s->mant, 0, bytes));
+      HIPCHK(s, hipMemset(s->mant, 0, tb));
   }
   s->ws_valid = false;
   s->list_valid = false;
@@ -735,6 +757,25 @@ int put_plane(hf2d_sim* s, void* dst, const double* aos, int f) {
     for (size_t k = 0; k < n; ++k) plane[k] = aos ? aos[3 * k + f] : 0.0;
     HIPCHK(s, hipMemcpy(dst, plane.data(), n * sizeof(double), hipMemcpyHostToDevice));
   }
+  return HF2D_OK;
+}
+// Tangents of field f between the AoS layout [cell][field][k] (k < n_tan)
+// and the device buffer [cell][k].
+int put_tangents(hf2d_sim* s, void* dst, const double* aos, int f) {
+  const size_t n = (size_t)(s->nr * s->nc), m = (size_t)s->n_tan;
+  std::vector<double> buf(n * m, 0.0);
+  if (aos)
+    for (size_t c = 0; c < n; ++c)
+      for (size_t k = 0; k < m; ++k) buf[c * m + k] = aos[(c * 3 + (size_t)f) * m + k];
+  HIPCHK(s, hipMemcpy(dst, buf.data(), n * m * sizeof(double), hipMemcpyHostToDevice));
+  return HF2D_OK;
+}
+int get_tangents(hf2d_sim* s, const void* src, double* aos, int f) {
+  const size_t n = (size_t)(s->nr * s->nc), m = (size_t)s->n_tan;
+  std::vector<double> buf(n * m);
+  HIPCHK(s, hipMemcpy(buf.data(), src, n * m * sizeof(double), hipMemcpyDeviceToHost));
+  for (size_t c = 0; c < n; ++c)
+    for (size_t k = 0; k < m; ++k) aos[(c * 3 + (size_t)f) * m + k] = buf[c * m + k];
   return HF2D_OK;
 }
 int get_plane(hf2d_sim* s, const void* src, double* aos, int f) {
@@ -758,8 +799,8 @@ int hf2d_upload_state(hf2d_sim* s, const double* aos, const double* tan_aos) {
   for (int f = 0; f < 3; ++f) {
     int rc = put_plane(s, s->st[s->cur][f], aos, f);
     if (rc) return rc;
-    if (s->kind == HF2D_DUAL) {
-      rc = put_plane(s, s->tg[s->cur][f], tan_aos, f);
+    if (s->n_tan > 0) {
+      rc = put_tangents(s, s->tg[s->cur][f], tan_aos, f);
       if (rc) return rc;
     }
   }
@@ -775,8 +816,8 @@ int hf2d_download_state(hf2d_sim* s, double* aos, double* tan_aos) {
   for (int f = 0; f < 3; ++f) {
     int rc = get_plane(s, s->st[s->cur][f], aos, f);
     if (rc) return rc;
-    if (s->kind == HF2D_DUAL && tan_aos) {
-      rc = get_plane(s, s->tg[s->cur][f], tan_aos, f);
+    if (s->n_tan > 0 && tan_aos) {
+      rc = get_tangents(s, s->tg[s->cur][f], tan_aos, f);
       if (rc) return rc;
     }
   }
@@ -789,9 +830,9 @@ int hf2d_total_volume(hf2d_sim* s, double* volume) {
   const int nb = 1024;
   double* part = nullptr;
   HIPCHK(s, hipMalloc(&part, 2 * nb * sizeof(double)));
-  if (s->kind == HF2D_DUAL)
-    hipLaunchKernelGGL(volume_kernel<dual>, dim3(nb), dim3(256), 0, s->stream,
-                       fld<dual>(s->st[s->cur][0], s->tg[s->cur][0]), n, part);
+  if (s->n_tan > 0)  // the value plane alone
+    hipLaunchKernelGGL(volume_kernel<double>, dim3(nb), dim3(256), 0, s->stream,
+                       fld<double>(s->st[s->cur][0], nullptr), n, part);
   else if (s->kind == HF2D_F32)
     hipLaunchKernelGGL(volume_kernel<float>, dim3(nb), dim3(256), 0, s->stream, fld<float>(s->st[s->cur][0], nullptr),
                        n, part);

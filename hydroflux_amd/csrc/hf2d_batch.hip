@@ -32,8 +32,8 @@ namespace {
 struct CfgInfo {
   int tw, th, nt;
 };
-// The autotuner's candidates per kind: the north-star shapes (256-cell tiles
-// 16x16, 32x8, 64x4, 128x2) and the larger high-occupancy tiles.
+// The autotuner's candidates per kind: the 256-cell tile shapes
+// (16x16, 32x8, 64x4, 128x2) and the larger high-occupancy tiles.
 const CfgInfo CFG_F32[] = {{32, 32, 1024}, {32, 16, 512}, {16, 16, 256}, {32, 8, 256}, {64, 4, 256}, {128, 2, 256}};
 const CfgInfo CFG_F64[] = {{32, 16, 1024}, {16, 16, 256}, {32, 8, 256}};
 const CfgInfo CFG_DUAL[] = {{16, 16, 256}, {16, 8, 256}};
@@ -508,6 +508,7 @@ const char* hf2d_batch_last_error(const hf2d_batch* b) { return b ? b->err.c_str
 
 int hf2d_batch_footprint(int32_t scalar_kind, int64_t n_rows, int64_t n_cols, int64_t n_sources, int64_t* bytes) {
   if (!bytes || n_rows <= 0 || n_cols <= 0) return HF2D_INVALID_ARGUMENT;
+  if (scalar_kind != HF2D_F64 && scalar_kind != HF2D_DUAL && scalar_kind != HF2D_F32) return HF2D_INVALID_ARGUMENT;
   *bytes = (int64_t)footprint(scalar_kind, n_rows, n_cols, n_sources);
   return HF2D_OK;
 }

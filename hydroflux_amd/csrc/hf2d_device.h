@@ -52,6 +52,17 @@ template <> struct Shape<double> {
 template <> struct Shape<dual> {
   static constexpr int TW = 16, TH = 16, NT = 256;
 };
+// DualN<N>: (N + 1) doubles per LDS value; tiles sized to stay within one
+// workgroup's LDS.
+template <> struct Shape<hf::dualn<2>> {
+  static constexpr int TW = 16, TH = 8, NT = 256;
+};
+template <> struct Shape<hf::dualn<4>> {
+  static constexpr int TW = 8, TH = 8, NT = 256;
+};
+template <> struct Shape<hf::dualn<8>> {
+  static constexpr int TW = 8, TH = 4, NT = 128;
+};
 #ifndef HF_F32_TILE_W
 #define HF_F32_TILE_W 32
 #endif
@@ -349,16 +360,29 @@ __global__ __launch_bounds__(256) void volume_kernel(Fld<T> h, int64_t n, double
 
 
 // ------------------------------------------------------------ host-side field views
-template <class T> Fld<T> fld(const void* v, const void* d);
-template <> Fld<double> fld<double>(const void* v, const void*) { return Fld<double>{(const double*)v}; }
-template <> Fld<dual> fld<dual>(const void* v, const void* d) {
-  return Fld<dual>{(const double*)v, (const double*)d};
-}
-template <> Fld<float> fld<float>(const void* v, const void*) { return Fld<float>{(const float*)v}; }
-template <class T> FldW<T> fldw(void* v, void* d);
-template <> FldW<double> fldw<double>(void* v, void*) { return FldW<double>{(double*)v}; }
-template <> FldW<dual> fldw<dual>(void* v, void* d) { return FldW<dual>{(double*)v, (double*)d}; }
-template <> FldW<float> fldw<float>(void* v, void*) { return FldW<float>{(float*)v}; }
+// Field views over raw device buffers: value plane v, tangent buffer d
+// (Dual: one per cell; DualN<N>: N per cell; null: tangent 0).
+template <class T> struct FldOf;
+template <> struct FldOf<double> {
+  static Fld<double> r(const void* v, const void*) { return Fld<double>{(const double*)v}; }
+  static FldW<double> w(void* v, void*) { return FldW<double>{(double*)v}; }
+};
+template <> struct FldOf<dual> {
+  static Fld<dual> r(const void* v, const void* d) { return Fld<dual>{(const double*)v, (const double*)d}; }
+  static FldW<dual> w(void* v, void* d) { return FldW<dual>{(double*)v, (double*)d}; }
+};
+template <> struct FldOf<float> {
+  static Fld<float> r(const void* v, const void*) { return Fld<float>{(const float*)v}; }
+  static FldW<float> w(void* v, void*) { return FldW<float>{(float*)v}; }
+};
+template <int N> struct FldOf<hf::dualn<N>> {
+  static Fld<hf::dualn<N>> r(const void* v, const void* d) {
+    return Fld<hf::dualn<N>>{(const double*)v, (const double*)d};
+  }
+  static FldW<hf::dualn<N>> w(void* v, void* d) { return FldW<hf::dualn<N>>{(double*)v, (double*)d}; }
+};
+template <class T> Fld<T> fld(const void* v, const void* d) { return FldOf<T>::r(v, d); }
+template <class T> FldW<T> fldw(void* v, void* d) { return FldOf<T>::w(v, d); }
 
 constexpr int DATUM_LOG2 = 6;  // FP32 kind: one FP64 bed datum per 64 x 64 cells
 

@@ -58,6 +58,17 @@ template <> struct Fld<dual> {
   const double* d;  // may be null: tangent 0 (constant bed)
   HF_HD dual ld(int64_t k) const { return dual{v[k], d ? d[k] : 0.0}; }
 };
+// DualN<N>: value plane, tangents per cell (d[cell * N + k]; null: 0).
+template <int N> struct Fld<dualn<N>> {
+  const double* v;
+  const double* d;
+  HF_HD dualn<N> ld(int64_t k) const {
+    dualn<N> r;
+    r.v = v[k];
+    for (int i = 0; i < N; ++i) r.d[i] = d ? d[k * N + i] : 0.0;
+    return r;
+  }
+};
 template <> struct Fld<float> {
   const float* v;
   HF_HD float ld(int64_t k) const { return v[k]; }
@@ -77,6 +88,14 @@ template <> struct FldW<dual> {
   HF_HD void st(int64_t k, dual x) const {
     v[k] = x.v;
     d[k] = x.d;
+  }
+};
+template <int N> struct FldW<dualn<N>> {
+  double* v;
+  double* d;
+  HF_HD void st(int64_t k, const dualn<N>& x) const {
+    v[k] = x.v;
+    for (int i = 0; i < N; ++i) d[k * N + i] = x.d[i];
   }
 };
 

@@ -39,10 +39,18 @@ HF_HD float val(float x) { return x; }
 HF_HD double rabs(double x) { return __builtin_fabs(x); }
 HF_HD float rabs(float x) { return __builtin_fabsf(x); }
 
-template <class T> HF_HD T cst(double x);
-template <> HF_HD double cst<double>(double x) { return x; }
-template <> HF_HD dual cst<dual>(double x) { return dual{x, 0.0}; }
-template <> HF_HD float cst<float>(double x) { return (float)x; }
+// cst<T>(x): the constant x as a T (tangents 0).
+template <class T> struct CstOf;
+template <> struct CstOf<double> {
+  static HF_HD double make(double x) { return x; }
+};
+template <> struct CstOf<dual> {
+  static HF_HD dual make(double x) { return dual{x, 0.0}; }
+};
+template <> struct CstOf<float> {
+  static HF_HD float make(double x) { return (float)x; }
+};
+template <class T> HF_HD T cst(double x) { return CstOf<T>::make(x); }
 
 HF_HD dual operator+(dual a, dual b) { return {a.v + b.v, a.d + b.d}; }
 HF_HD dual operator-(dual a, dual b) { return {a.v - b.v, a.d - b.d}; }
@@ -108,5 +116,88 @@ HF_HD dual pow35(dual x) {
   return {hfm::pow_3_5(x.v), n * hfm::pow_m2_5(x.v) * x.d};
 }
 HF_HD float pow35(float x) { return powf(x, 0.6f); }
+
+// ------------------------------------------------------------ DualN<N>
+//
+// The reference's vector-mode dual (autograd/src/dual_n.rs, Real impl at
+// real.rs:161-195): a value and N tangents. Its rules are not Dual's:
+// every binary operation is combine(a, b, val, ca, cb) with tangents
+// ca * a_i + cb * b_i, every unary one chain(val, dfdx) with a_i * dfdx;
+// division takes inv = 1 / b and the value a * inv; sqrt's factor is
+// 0.5 / s (0 at s == 0); max/min ties average as 0.5 * a_i + 0.5 * b_i;
+// an f64 operand is promoted to a constant (tangents 0) first
+// (dual_n.rs scalar_ops). Branches read the value alone.
+template <int N> struct dualn {
+  double v;
+  double d[N];
+};
+
+template <int N> HF_HD dualn<N> dn_const(double x) {
+  dualn<N> r;
+  r.v = x;
+  for (int i = 0; i < N; ++i) r.d[i] = 0.0;
+  return r;
+}
+template <int N> HF_HD dualn<N> dn_combine(const dualn<N>& a, const dualn<N>& b, double val, double ca, double cb) {
+  dualn<N> r;
+  r.v = val;
+  for (int i = 0; i < N; ++i) r.d[i] = ca * a.d[i] + cb * b.d[i];
+  return r;
+}
+template <int N> HF_HD dualn<N> dn_chain(const dualn<N>& a, double val, double dfdx) {
+  dualn<N> r;
+  r.v = val;
+  for (int i = 0; i < N; ++i) r.d[i] = a.d[i] * dfdx;
+  return r;
+}
+
+template <int N> struct CstOf<dualn<N>> {
+  static HF_HD dualn<N> make(double x) { return dn_const<N>(x); }
+};
+
+template <int N> HF_HD double val(const dualn<N>& x) { return x.v; }
+template <int N> HF_HD dualn<N> operator+(const dualn<N>& a, const dualn<N>& b) {
+  return dn_combine(a, b, a.v + b.v, 1.0, 1.0);
+}
+template <int N> HF_HD dualn<N> operator-(const dualn<N>& a, const dualn<N>& b) {
+  return dn_combine(a, b, a.v - b.v, 1.0, -1.0);
+}
+template <int N> HF_HD dualn<N> operator*(const dualn<N>& a, const dualn<N>& b) {
+  return dn_combine(a, b, a.v * b.v, b.v, a.v);
+}
+template <int N> HF_HD dualn<N> operator/(const dualn<N>& a, const dualn<N>& b) {
+  const double inv = 1.0 / b.v;
+  return dn_combine(a, b, a.v * inv, inv, -a.v * inv * inv);
+}
+template <int N> HF_HD dualn<N> operator-(const dualn<N>& a) { return dn_chain(a, -a.v, -1.0); }
+template <int N> HF_HD dualn<N> operator+(const dualn<N>& a, double b) { return a + dn_const<N>(b); }
+template <int N> HF_HD dualn<N> operator-(const dualn<N>& a, double b) { return a - dn_const<N>(b); }
+template <int N> HF_HD dualn<N> operator*(const dualn<N>& a, double b) { return a * dn_const<N>(b); }
+template <int N> HF_HD dualn<N> operator/(const dualn<N>& a, double b) { return a / dn_const<N>(b); }
+template <int N> HF_HD dualn<N> div(const dualn<N>& a, const dualn<N>& b) { return a / b; }
+template <int N> HF_HD dualn<N> div(const dualn<N>& a, double b) { return a / dn_const<N>(b); }
+template <int N> HF_HD dualn<N> rmax(const dualn<N>& a, const dualn<N>& b) {
+  if (a.v > b.v) return a;
+  if (b.v > a.v) return b;
+  return dn_combine(a, b, a.v, 0.5, 0.5);
+}
+template <int N> HF_HD dualn<N> rmin(const dualn<N>& a, const dualn<N>& b) {
+  if (a.v < b.v) return a;
+  if (b.v < a.v) return b;
+  return dn_combine(a, b, a.v, 0.5, 0.5);
+}
+template <int N> HF_HD dualn<N> rsqrt_(const dualn<N>& a) {
+  const double s = __builtin_sqrt(a.v);
+  return dn_chain(a, s, s == 0.0 ? 0.0 : 0.5 / s);
+}
+// powf(4/3) and powf(3/5) with the deterministic powers: chain(v^n, n v^(n-1))
+template <int N> HF_HD dualn<N> pow43(const dualn<N>& x) {
+  const double n = 4.0 / 3.0;
+  return dn_chain(x, hfm::pow_4_3(x.v), n * hfm::pow_1_3m(x.v));
+}
+template <int N> HF_HD dualn<N> pow35(const dualn<N>& x) {
+  const double n = 3.0 / 5.0;
+  return dn_chain(x, hfm::pow_3_5(x.v), n * hfm::pow_m2_5(x.v));
+}
 
 }  // namespace hf

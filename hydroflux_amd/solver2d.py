@@ -212,7 +212,8 @@ class Grid2D:
     device; ``set_state``/``get_state`` are the only host<->device copies.
     """
 
-    KINDS = {"f64": 0, "dual": 1, "f32": 2}
+    KINDS = {"f64": 0, "dual": 1, "f32": 2, "dual2": 3, "dual4": 4, "dual8": 5}
+    TANGENTS = {"f64": 0, "dual": 1, "f32": 0, "dual2": 2, "dual4": 4, "dual8": 8}
 
     def __init__(self, n_rows: int, n_cols: int, dx: float, dy: float, dual: bool = False, kind: str = None):
         kind = kind or ("dual" if dual else "f64")
@@ -224,8 +225,14 @@ class Grid2D:
         self._h = h
         self.n_rows, self.n_cols, self.dx, self.dy = int(n_rows), int(n_cols), float(dx), float(dy)
         self.kind = kind
-        self.dual = kind == "dual"
+        self.n_tan = self.TANGENTS[kind]
+        self.dual = self.n_tan > 0
         self._n_gauges = 0
+
+    def _tshape(self, fields=None):
+        """Tangent array shape: Dual like the values; DualN<N> with a last axis N."""
+        base = (self.n_rows, self.n_cols) if fields is None else (self.n_rows, self.n_cols, fields)
+        return base if self.n_tan == 1 else base + (self.n_tan,)
 
     @classmethod
     def for_mesh(cls, mesh: Mesh2D, dual: Optional[bool] = None) -> "Grid2D":
@@ -252,8 +259,10 @@ class Grid2D:
     # -- data movement
     def set_mesh(self, bed, manning, bed_tangent=None, manning_tangent=None):
         b, m = self._grid(bed), self._grid(manning)
-        bt = None if bed_tangent is None else self._grid(bed_tangent)
-        mt = None if manning_tangent is None else self._grid(manning_tangent)
+        tan = (lambda x: np.ascontiguousarray(np.broadcast_to(np.asarray(x, np.float64), self._tshape()))) \
+            if self.n_tan > 1 else self._grid
+        bt = None if bed_tangent is None else tan(bed_tangent)
+        mt = None if manning_tangent is None else tan(manning_tangent)
         _check(lib().hf2d_upload_mesh(self._h, _p(b), _p(bt), _p(m), _p(mt)), self._h)
 
     def set_state(self, state, tangent=None):
@@ -263,13 +272,13 @@ class Grid2D:
         t = None
         if tangent is not None:
             t = np.ascontiguousarray(tangent, np.float64)
-            if t.shape != s.shape:
-                raise ShapeMismatch("tangent shape must match the state")
+            if t.shape != self._tshape(3):
+                raise ShapeMismatch(f"tangent shape must be {self._tshape(3)}")
         _check(lib().hf2d_upload_state(self._h, _p(s), _p(t)), self._h)
 
     def get_state(self):
         s = np.empty((self.n_rows, self.n_cols, 3), np.float64)
-        t = np.empty_like(s) if self.dual else None
+        t = np.empty(self._tshape(3)) if self.dual else None
         _check(lib().hf2d_download_state(self._h, _p(s), _p(t)), self._h)
         return (s, t) if self.dual else s
 

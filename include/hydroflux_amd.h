@@ -39,11 +39,25 @@ enum hf2d_status {
   HF2D_CAPACITY = 7                /* batch: no free slot or the scenario does not fit in HBM */
 };
 
-/* Scalar of the grid: f64, forward-mode Dual (autograd/src/dual.rs:12), or the
+/* Scalar of the grid: f64, forward-mode Dual (autograd/src/dual.rs:12), the
  * FP32 precision form (no reference counterpart: FP32 state and flux
  * arithmetic, bed as an FP64 datum per 64x64 block + FP32 deviation, FP64
- * dt reduction and Kahan-compensated clock). */
-enum hf2d_scalar_kind { HF2D_F64 = 0, HF2D_DUAL = 1, HF2D_F32 = 2 };
+ * dt reduction and Kahan-compensated clock), or the vector-mode DualN<N>
+ * (autograd/src/dual_n.rs, Real impl real.rs:161-195) for N = 2, 4, 8
+ * derivatives in one pass, with the reference's DualN arithmetic.
+ *
+ * Tangent arrays at the boundary: Dual, one per value (same layout as the
+ * values); DualN<N>, N per value, innermost: the state tangents are
+ * [cell][field][k] (n_cells x 3 x N), the bed and Manning tangents
+ * [cell][k] (n_cells x N). The batch API takes f64, Dual and FP32. */
+enum hf2d_scalar_kind {
+  HF2D_F64 = 0,
+  HF2D_DUAL = 1,
+  HF2D_F32 = 2,
+  HF2D_DUAL2 = 3,
+  HF2D_DUAL4 = 4,
+  HF2D_DUAL8 = 5
+};
 
 /* Boundary kinds (boundary.rs:80-106). Sides are ordered N, S, W, E
  * (Boundaries2D, boundary.rs:131-140). param is q for Discharge, h for

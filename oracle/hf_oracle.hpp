@@ -129,6 +129,82 @@ inline Dual rpowf(Dual a, double n) {
 }
 template <> inline Dual from_f64<Dual>(double x) { return {x, 0.0}; }
 
+// Vector-mode dual, autograd/src/dual_n.rs (Real impl real.rs:161-195): a
+// value and N tangents. combine(o, val, a, b) sets tangent i to
+// a * self_i + b * o_i; chain(val, dfdx) to self_i * dfdx; an f64 operand
+// is a constant DualN (scalar_ops). Division: inv = 1 / o.val, value
+// self.val * inv, factors (inv, -self.val * inv * inv).
+template <int N> struct DualN {
+  double val;
+  double dval[N];
+};
+template <int N> inline DualN<N> dn_constant(double x) {
+  DualN<N> r;
+  r.val = x;
+  for (int i = 0; i < N; ++i) r.dval[i] = 0.0;
+  return r;
+}
+template <int N> inline DualN<N> dn_combine(const DualN<N>& s, const DualN<N>& o, double val, double a, double b) {
+  DualN<N> r;
+  r.val = val;
+  for (int i = 0; i < N; ++i) r.dval[i] = a * s.dval[i] + b * o.dval[i];
+  return r;
+}
+template <int N> inline DualN<N> dn_chain(const DualN<N>& s, double val, double dfdx) {
+  DualN<N> r;
+  r.val = val;
+  for (int i = 0; i < N; ++i) r.dval[i] = s.dval[i] * dfdx;
+  return r;
+}
+template <int N> inline double value(const DualN<N>& d) { return d.val; }
+template <int N> inline DualN<N> operator+(const DualN<N>& s, const DualN<N>& o) {
+  return dn_combine(s, o, s.val + o.val, 1.0, 1.0);
+}
+template <int N> inline DualN<N> operator-(const DualN<N>& s, const DualN<N>& o) {
+  return dn_combine(s, o, s.val - o.val, 1.0, -1.0);
+}
+template <int N> inline DualN<N> operator*(const DualN<N>& s, const DualN<N>& o) {
+  return dn_combine(s, o, s.val * o.val, o.val, s.val);
+}
+// Value of a DualN quotient: 0 the reference's s.val * (1 / o.val); 1 (a
+// test switch, not the reference) s.val / o.val, f64's value, so that a
+// DualN run follows the f64 value path exactly and its tangents can be
+// compared with seeded Dual runs to rounding.
+inline int& dualn_div_mode() {
+  static int m = 0;
+  return m;
+}
+template <int N> inline DualN<N> operator/(const DualN<N>& s, const DualN<N>& o) {
+  const double inv = 1.0 / o.val;
+  return dn_combine(s, o, dualn_div_mode() ? s.val / o.val : s.val * inv, inv, -s.val * inv * inv);
+}
+template <int N> inline DualN<N> operator-(const DualN<N>& s) { return dn_chain(s, -s.val, -1.0); }
+template <int N> inline DualN<N> operator+(const DualN<N>& s, double x) { return s + dn_constant<N>(x); }
+template <int N> inline DualN<N> operator-(const DualN<N>& s, double x) { return s - dn_constant<N>(x); }
+template <int N> inline DualN<N> operator*(const DualN<N>& s, double x) { return s * dn_constant<N>(x); }
+template <int N> inline DualN<N> operator/(const DualN<N>& s, double x) { return s / dn_constant<N>(x); }
+template <int N> inline DualN<N> rmax(const DualN<N>& s, const DualN<N>& o) {  // dual_n.rs max
+  if (s.val > o.val) return s;
+  if (o.val > s.val) return o;
+  return dn_combine(s, o, s.val, 0.5, 0.5);
+}
+template <int N> inline DualN<N> rmin(const DualN<N>& s, const DualN<N>& o) {
+  if (s.val < o.val) return s;
+  if (o.val < s.val) return o;
+  return dn_combine(s, o, s.val, 0.5, 0.5);
+}
+template <int N> inline DualN<N> rsqrt_(const DualN<N>& s) {  // dual_n.rs sqrt
+  const double r = std::sqrt(s.val);
+  return dn_chain(s, r, r == 0.0 ? 0.0 : 0.5 / r);
+}
+template <int N> inline DualN<N> rpowf(const DualN<N>& s, double n) {  // dual_n.rs powf
+  return dn_chain(s, powf64(s.val, n), n * powf64(s.val, n - 1.0));
+}
+template <> inline DualN<2> from_f64<DualN<2>>(double x) { return dn_constant<2>(x); }
+template <> inline DualN<3> from_f64<DualN<3>>(double x) { return dn_constant<3>(x); }
+template <> inline DualN<4> from_f64<DualN<4>>(double x) { return dn_constant<4>(x); }
+template <> inline DualN<8> from_f64<DualN<8>>(double x) { return dn_constant<8>(x); }
+
 template <class T> inline T zero() { return from_f64<T>(0.0); }
 template <class T> inline T one() { return from_f64<T>(1.0); }
 

@@ -64,6 +64,7 @@ def lib():
             getattr(L, name).argtypes = [C.c_double]
         L.hfo_libm_pow.restype = C.c_double
         L.hfo_libm_pow.argtypes = [C.c_double, C.c_double]
+        L.hfo_dualn_probe.argtypes = [C.c_double, C.c_double, C.c_double, _D]
         _LIB = L
     return _LIB
 
@@ -81,13 +82,37 @@ def set_threads(n: int) -> None:
     lib().hfo_set_threads(int(n))
 
 
-class OracleSim:
-    """One grid on the CPU restatement. States are (n_rows, n_cols, 3) float64."""
+def set_dualn_div_mode(mode: int) -> None:
+    """DualN quotient value: 0 the reference's a * (1 / b); 1 a / b (a
+    test switch that makes DualN follow the f64 value path)."""
+    lib().hfo_set_dualn_div_mode(int(mode))
 
-    def __init__(self, n_rows, n_cols, dx, dy, dual=False):
+
+def dualn_probe(a, b, c):
+    """The reference's DualN<3> unit-test function and its three Dual passes:
+    (vec value, vec tangents[3], [(dual value, dual tangent)] * 3)."""
+    out = np.zeros(10)
+    lib().hfo_dualn_probe(float(a), float(b), float(c), _p(out))
+    return out[0], out[1:4], [(out[4 + 2 * k], out[5 + 2 * k]) for k in range(3)]
+
+
+class OracleSim:
+    """One grid on the CPU restatement. States are (n_rows, n_cols, 3) float64;
+    Dual tangents the same shape; DualN<N> (tangents=N in 2, 4, 8) tangents
+    (n_rows, n_cols, 3, N) for the state and (n_rows, n_cols, N) for bed and
+    Manning."""
+
+    def __init__(self, n_rows, n_cols, dx, dy, dual=False, tangents=None):
         self.n_rows, self.n_cols, self.dx, self.dy = int(n_rows), int(n_cols), float(dx), float(dy)
-        self.dual = bool(dual)
-        self._h = lib().hfo_create(self.n_rows, self.n_cols, self.dx, self.dy, int(self.dual))
+        if tangents is not None and tangents not in (2, 4, 8):
+            raise ValueError("tangents must be 2, 4 or 8")
+        self.n_tan = int(tangents) if tangents else (1 if dual else 0)
+        self.dual = self.n_tan > 0
+        self._h = lib().hfo_create(self.n_rows, self.n_cols, self.dx, self.dy, self.n_tan)
+
+    def _tshape(self, fields=None):
+        base = (self.n_rows, self.n_cols) if fields is None else (self.n_rows, self.n_cols, fields)
+        return base if self.n_tan == 1 else base + (self.n_tan,)
 
     def close(self):
         if self._h:
@@ -105,20 +130,20 @@ class OracleSim:
         bed = np.ascontiguousarray(np.broadcast_to(np.asarray(bed, np.float64), (self.n_rows, self.n_cols)))
         man = np.ascontiguousarray(np.broadcast_to(np.asarray(manning, np.float64), (self.n_rows, self.n_cols)))
         bt = None if bed_tangent is None else np.ascontiguousarray(
-            np.broadcast_to(np.asarray(bed_tangent, np.float64), (self.n_rows, self.n_cols)))
+            np.broadcast_to(np.asarray(bed_tangent, np.float64), self._tshape()))
         mt = None if manning_tangent is None else np.ascontiguousarray(
-            np.broadcast_to(np.asarray(manning_tangent, np.float64), (self.n_rows, self.n_cols)))
+            np.broadcast_to(np.asarray(manning_tangent, np.float64), self._tshape()))
         assert bed.size == n
         lib().hfo_set_mesh(self._h, _p(bed), _p(bt), _p(man), _p(mt))
 
     def set_state(self, state, tangent=None):
         s = np.ascontiguousarray(state, np.float64).reshape(self.n_rows, self.n_cols, 3)
-        t = None if tangent is None else np.ascontiguousarray(tangent, np.float64).reshape(s.shape)
+        t = None if tangent is None else np.ascontiguousarray(tangent, np.float64).reshape(self._tshape(3))
         lib().hfo_set_state(self._h, _p(s), _p(t))
 
     def get_state(self):
         s = np.empty((self.n_rows, self.n_cols, 3), np.float64)
-        t = np.empty_like(s) if self.dual else None
+        t = np.empty(self._tshape(3)) if self.dual else None
         lib().hfo_get_state(self._h, _p(s), _p(t))
         return (s, t) if self.dual else s
 

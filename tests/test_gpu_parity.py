@@ -212,6 +212,39 @@ def test_graph_replays_interleaved_with_host_calls_bitwise(rain):
     assert g.time() == o.time() and np.array_equal(g.get_state(), o.get_state())
 
 
+@pytest.mark.parametrize("n", [2, 4, 8])
+def test_dualn_bitwise(n):
+    """Vector-mode DualN<N> (autograd/src/dual_n.rs) on the GPU: the fused
+    driver with rain and point sources, then a host-side loop of
+    cfl_time_step / ssprk2_step / friction; values and all N tangents (bed,
+    Manning and state seeds) bit for bit with the oracle."""
+    _say(f"dualn<{n}>")
+    c = cases.bumpy_wet_dry(41, 57, 12)
+    c.manning = np.maximum(c.manning, 0.03)
+    rng = np.random.default_rng(n)
+    kw = dict(manning_tangent=0.01 * rng.standard_normal((c.nr, c.nc, n)),
+              bed_tangent=0.01 * rng.standard_normal((c.nr, c.nc, n)),
+              state_tangent=0.1 * rng.standard_normal((c.nr, c.nc, 3, n)))
+    g = c.load(_hf().Grid2D(c.nr, c.nc, c.dx, c.dy, kind=f"dual{n}"), **kw)
+    o = c.load(OracleSim(c.nr, c.nc, c.dx, c.dy, tangents=n), **kw)
+    for s in (g, o):
+        s.configure(cfl=0.4, max_dt=0.05, rain_rate=2e-4, sources=[(5, 7, 0.2)])
+        s.run_steps(20)
+    assert g.time() == o.time()
+    (gv, gt), (ov, ot) = g.get_state(), o.get_state()
+    assert np.array_equal(gv, ov) and np.array_equal(gt, ot, equal_nan=True)
+    for _ in range(10):
+        dg, do = g.cfl_time_step(0.4), o.cfl_time_step(0.4)
+        assert dg == do
+        g.ssprk2_step(dg)
+        o.ssprk2_step(do)
+        g.manning_friction_step(dg, 1e-6)
+        o.manning_friction_step(do, 1e-6)
+    (gv, gt), (ov, ot) = g.get_state(), o.get_state()
+    assert gt.shape == (c.nr, c.nc, 3, n)
+    assert np.array_equal(gv, ov) and np.array_equal(gt, ot, equal_nan=True)
+
+
 def test_forward_euler_driver_bitwise():
     _say("forward euler driver")
     c = cases.bumpy_wet_dry(40, 70, 5)
