@@ -297,6 +297,87 @@ def uk_ea(args):
     return out
 
 
+def _zonal_pass(n, steps, kind, n_params, seed_zone=None):
+    """solver-2d/examples/m1_forward_scaling.rs on the GPU: dam break (h = 1
+    left half, 0.1 right half), transmissive edges, Manning 0.03 in
+    n_params vertical stripes, `steps` SSP-RK2 + friction(1e-6) steps at a
+    fixed dt of 0.01 s, host loop as in the example. Returns the wall time
+    and the depth tangent field (Dual: n x n; DualN<P>: n x n x P)."""
+    from hydroflux_amd import Grid2D
+
+    st = np.zeros((n, n, 3))
+    st[..., 0] = np.where(np.arange(n) < n // 2, 1.0, 0.1)[None, :]
+    zone = (np.arange(n) * n_params) // n
+    g = Grid2D(n, n, 1.0, 1.0, kind=kind)
+    mt = None
+    if kind == "dual":
+        mt = np.where(zone == seed_zone, 1.0, 0.0)[None, :].repeat(n, 0)
+    elif kind != "f64":
+        mt = np.zeros((n, n, n_params))
+        for k in range(n_params):
+            mt[:, zone == k, k] = 1.0
+    g.set_mesh(np.zeros((n, n)), np.full((n, n), 0.03), manning_tangent=mt)
+    g.set_state(st)
+    g.set_boundaries([0, 0, 0, 0])
+    g.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        g.ssprk2_step(0.01)
+        g.manning_friction_step(0.01, 1e-6)
+    g.synchronize()
+    wall = time.perf_counter() - t0
+    out = g.get_state()
+    g.close()
+    if kind == "f64":
+        return wall, None
+    return wall, out[1][..., 0] if kind == "dual" else out[1][..., 0, :]
+
+
+def forward_scaling(args):
+    """Cost of the full gradient over P zonal Manning parameters relative to
+    the f64 primal, as solver-2d/examples/m1_forward_scaling.rs defines it:
+    P seeded Dual passes (per-parameter cost = cost / primal / P) and one
+    DualN<P> pass (per-parameter cost = (cost / primal - 1) / P), on the
+    example's 64^2 x 200-step workload and on the same construction at
+    2048^2 x 20 steps. The reference's figures on its CPU: 2.05-2.18x per
+    parameter with Dual (papers/01_review/figures/data/m1_forward_scaling.csv:2-6),
+    0.88x with DualN (papers/01_review/manuscript.md:406-412).
+    tangent_rel_diff: max over the P components of the DualN depth tangent
+    field against the seeded Dual run's, relative to the latter's largest
+    entry (the two value paths differ in the last bit, so cells where a
+    near-tie resolves differently differ)."""
+    out = {}
+    for n, steps in ((64, 200), (2048, 20)):
+        _zonal_pass(n, steps, "f64", 1)  # warm
+        prim = min(_zonal_pass(n, steps, "f64", 1)[0] for _ in range(3))
+        rows = []
+        for p in (1, 2, 4, 8, 16):
+            fields, wall_d = [], 0.0
+            for k in range(p):
+                w, td = _zonal_pass(n, steps, "dual", p, k)
+                wall_d += w
+                fields.append(td)
+            row = {"P": p, "dual_passes_s": wall_d, "dual_cost_vs_primal": wall_d / prim,
+                   "dual_per_param": wall_d / prim / p}
+            if p > 1:
+                _zonal_pass(n, steps, f"dual{p}", p)  # warm
+                w, tn = _zonal_pass(n, steps, f"dual{p}", p)
+                diff = max(float(np.max(np.abs(tn[..., k] - fields[k]))) / max(float(np.max(np.abs(fields[k]))),
+                                                                                1e-300) for k in range(p))
+                row.update({"dualn_s": w, "dualn_cost_vs_primal": w / prim, "dualn_per_param": (w / prim - 1.0) / p,
+                            "tangent_rel_diff": diff})
+            rows.append(row)
+            log(f"[bench] forward scaling {n}^2 P={p}: Dual {row['dual_per_param']:.2f}x / param"
+                + (f", DualN {row['dualn_per_param']:.2f}x / param" if p > 1 else ""))
+        out[f"{n}x{n}x{steps}"] = {
+            "primal_s": prim, "rows": rows,
+            "dual_per_param_mean": float(np.mean([r["dual_per_param"] for r in rows])),
+            "dualn_per_param_mean": float(np.mean([r["dualn_per_param"] for r in rows if "dualn_per_param" in r]))}
+    out["reference_cpu"] = {"grid": "64x64x200", "dual_per_param": [2.0481, 2.1392, 2.1777, 2.1176, 2.1796],
+                            "dualn_per_param_mean": 0.88}
+    return out
+
+
 def batch_bench(args):
     from hydroflux_amd import Grid2D
     from hydroflux_amd.batch import Batch, ScenarioSpec
@@ -393,8 +474,8 @@ def main():
     ap.add_argument("--size", type=int, default=4096)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="headline FP64 line only (and cpu_baseline)")
-    ap.add_argument("--sections", default="precision,c1,dry,ukea,batch,serving",
-                    help="comma list of extra sections: precision, c1, dry, ukea, batch, serving")
+    ap.add_argument("--sections", default="precision,c1,dry,ukea,forward,batch,serving",
+                    help="comma list of extra sections: precision, c1, dry, ukea, forward, batch, serving")
     ap.add_argument("--dry-warmup", type=int, default=500)
     ap.add_argument("--acc-steps", type=int, default=2000, help="FP64 steps for the FP32 accuracy check (0: skip)")
     ap.add_argument("--serve-requests", type=int, default=64)
@@ -466,6 +547,8 @@ def main():
         out["dry_domain"] = dry_domain(args)
     if "ukea" in sections:
         out["uk_ea"] = uk_ea(args)
+    if "forward" in sections:
+        out["forward_scaling"] = forward_scaling(args)
     if "batch" in sections or "serving" in sections:
         out["batch"], mix = batch_bench(args)
         if "serving" in sections:

@@ -160,8 +160,58 @@ template <class T> StageArgs<T> make_args(hf2d_sim* s, int in, int out, int un) 
   return a;
 }
 
-template <class T> int launch_stage(hf2d_sim* s, StageArgs<T> a, int mode, int timing_slot) {
+template <class T> struct WideDualN {
+  static constexpr int groups = 0;
+};
+template <int N> struct WideDualN<hf::dualn<N>> {
+  static constexpr int groups = N > 4 ? N / 4 : 0;
+};
+
+// Component group c (tangents [4c, 4c + 4)) of a DualN<N> launch as a
+// DualN<4> launch. Every group computes the same values, bit for bit (a
+// DualN value never depends on its tangents, and each tangent component
+// follows the same rule whatever N is); only the last group stores values
+// and reduces the CFL wave speeds, so the in-place final stage reads U^n
+// values that no earlier group has overwritten.
+template <int N> StageArgs<hf::dualn<4>> group_args(const StageArgs<hf::dualn<N>>& a, int c) {
+  using G = hf::dualn<4>;
+  static_assert(Shape<hf::dualn<N>>::TW == Shape<G>::TW && Shape<hf::dualn<N>>::TH == Shape<G>::TH,
+                "tiles of a DualN<N> launch are the DualN<4> kernel's");
+  const bool last = c == N / 4 - 1;
+  StageArgs<G> b{};
+  static_cast<hf::StageCommon&>(b) = a;
+  auto r = [&](const Fld<hf::dualn<N>>& f) { return Fld<G>{f.v, f.d ? f.d + 4 * c : nullptr, f.ds}; };
+  auto w = [&](const FldW<hf::dualn<N>>& f) { return FldW<G>{f.v, f.d + 4 * c, f.ds, last ? f.wv : 0}; };
+  b.ih = r(a.ih);
+  b.ihu = r(a.ihu);
+  b.ihv = r(a.ihv);
+  b.oh = w(a.oh);
+  b.ohu = w(a.ohu);
+  b.ohv = w(a.ohv);
+  b.nh = r(a.nh);
+  b.nhu = r(a.nhu);
+  b.nhv = r(a.nhv);
+  b.z = r(a.z);
+  b.man = r(a.man);
+  if (!last) {
+    b.wave_speed = 0;
+    b.tile_wet = nullptr;
+  }
+  return b;
+}
+
+template <class T> int launch_stage_kernel(hf2d_sim* s, const StageArgs<T>& a, int mode) {
   const unsigned grid = (unsigned)((int64_t)a.tiles_x * a.tiles_y);
+  if (mode == hf::MODE_STAGE)
+    hipLaunchKernelGGL((stage_kernel<T, hf::MODE_STAGE>), dim3(grid), dim3(Shape<T>::NT), 0, s->stream, a);
+  else
+    hipLaunchKernelGGL((stage_kernel<T, hf::MODE_FINAL_RK2>), dim3(grid), dim3(Shape<T>::NT), 0, s->stream,
+                       a);
+  HIPCHK(s, hipGetLastError());
+  return HF2D_OK;
+}
+
+template <class T> int launch_stage(hf2d_sim* s, StageArgs<T> a, int mode, int timing_slot) {
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (s->timing && timing_slot >= 0) {
     if (s->ev_used + 2 > s->ev.size()) {
@@ -176,12 +226,15 @@ template <class T> int launch_stage(hf2d_sim* s, StageArgs<T> a, int mode, int t
     s->ev_used += 2;
     HIPCHK(s, hipEventRecord(e0, s->stream));
   }
-  if (mode == hf::MODE_STAGE)
-    hipLaunchKernelGGL((stage_kernel<T, hf::MODE_STAGE>), dim3(grid), dim3(Shape<T>::NT), 0, s->stream, a);
-  else
-    hipLaunchKernelGGL((stage_kernel<T, hf::MODE_FINAL_RK2>), dim3(grid), dim3(Shape<T>::NT), 0, s->stream,
-                       a);
-  HIPCHK(s, hipGetLastError());
+  if constexpr (WideDualN<T>::groups > 0) {
+    for (int c = 0; c < WideDualN<T>::groups; ++c) {
+      const int rc = launch_stage_kernel(s, group_args(a, c), mode);
+      if (rc) return rc;
+    }
+  } else {
+    const int rc = launch_stage_kernel(s, a, mode);
+    if (rc) return rc;
+  }
   if (e1) {
     HIPCHK(s, hipEventRecord(e1, s->stream));
     (void)timing_slot;
@@ -582,6 +635,7 @@ int upload_pins(hf2d_sim* s, const hf2d_point_source* src, int64_t k) {
 using dual2 = hf::dualn<2>;
 using dual4 = hf::dualn<4>;
 using dual8 = hf::dualn<8>;
+using dual16 = hf::dualn<16>;
 
 static int tangents_of(int kind) {
   switch (kind) {
@@ -589,6 +643,7 @@ static int tangents_of(int kind) {
     case HF2D_DUAL2: return 2;
     case HF2D_DUAL4: return 4;
     case HF2D_DUAL8: return 8;
+    case HF2D_DUAL16: return 16;
     default: return 0;
   }
 }
@@ -598,6 +653,7 @@ static int tangents_of(int kind) {
    : (s)->kind == HF2D_DUAL2 ? FN<dual2>(__VA_ARGS__)             \
    : (s)->kind == HF2D_DUAL4 ? FN<dual4>(__VA_ARGS__)             \
    : (s)->kind == HF2D_DUAL8 ? FN<dual8>(__VA_ARGS__)             \
+   : (s)->kind == HF2D_DUAL16 ? FN<dual16>(__VA_ARGS__)           \
    : (s)->kind == HF2D_F32 ? FN<float>(__VA_ARGS__)               \
                            : FN<double>(__VA_ARGS__))
 
@@ -921,8 +977,8 @@ int hf2d_apply_point_sources(hf2d_sim* s, const hf2d_point_source* src, int64_t 
   if (rc == HF2D_OK) {
     HF_DISPATCH(s, launch_point_sources, s, dt);
     hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);  // the list is freed below
     if (e != hipSuccess) rc = fail(s, HF2D_HIP_ERROR, hipGetErrorString(e));
-    hipStreamSynchronize(s->stream);
   }
   if (s->srcs) (void)hipFree(s->srcs);
   s->srcs = saved;
@@ -1159,17 +1215,19 @@ int hf2d_math_probe(const double* x, const double* y, int64_t n, double* out) {
   if (!x || !y || !out || n <= 0) return HF2D_INVALID_ARGUMENT;
   double *dx_ = nullptr, *dy_ = nullptr, *dout = nullptr;
   const size_t b = (size_t)n * sizeof(double);
-  if (hipMalloc(&dx_, b) != hipSuccess) return HF2D_HIP_ERROR;
-  if (hipMalloc(&dy_, b) != hipSuccess) return HF2D_HIP_ERROR;
-  if (hipMalloc(&dout, 4 * b) != hipSuccess) return HF2D_HIP_ERROR;
-  hipMemcpy(dx_, x, b, hipMemcpyHostToDevice);
-  hipMemcpy(dy_, y, b, hipMemcpyHostToDevice);
-  hipLaunchKernelGGL(math_probe_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, 0, dx_, dy_, n, dout);
-  hipError_t e = hipGetLastError();
+  hipError_t e = hipMalloc(&dx_, b);
+  if (e == hipSuccess) e = hipMalloc(&dy_, b);
+  if (e == hipSuccess) e = hipMalloc(&dout, 4 * b);
+  if (e == hipSuccess) e = hipMemcpy(dx_, x, b, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(dy_, y, b, hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(math_probe_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, 0, dx_, dy_, n, dout);
+    e = hipGetLastError();
+  }
   if (e == hipSuccess) e = hipMemcpy(out, dout, 4 * b, hipMemcpyDeviceToHost);
-  hipFree(dx_);
-  hipFree(dy_);
-  hipFree(dout);
+  if (dx_) (void)hipFree(dx_);
+  if (dy_) (void)hipFree(dy_);
+  if (dout) (void)hipFree(dout);
   return e == hipSuccess ? HF2D_OK : HF2D_HIP_ERROR;
 }
 

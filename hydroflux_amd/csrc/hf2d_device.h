@@ -53,15 +53,19 @@ template <> struct Shape<dual> {
   static constexpr int TW = 16, TH = 16, NT = 256;
 };
 // DualN<N>: (N + 1) doubles per LDS value; tiles sized to stay within one
-// workgroup's LDS.
+// workgroup's LDS. For N > 4 the stage kernels run as N / 4 launches of the
+// DualN<4> kernel, one per group of 4 tangent components (a DualN<8> tile
+// of its own would be 8 x 4 cells at 1 workgroup of 2 waves per CU, with 4.4
+// cells loaded per cell updated), so its tiles are DualN<4>'s.
+template <int N> struct Shape<hf::dualn<N>> {
+  static_assert(N > 4 && N % 4 == 0, "DualN<N> wider than 4 runs in groups of 4 components");
+  static constexpr int TW = 8, TH = 8, NT = 256;
+};
 template <> struct Shape<hf::dualn<2>> {
   static constexpr int TW = 16, TH = 8, NT = 256;
 };
 template <> struct Shape<hf::dualn<4>> {
   static constexpr int TW = 8, TH = 8, NT = 256;
-};
-template <> struct Shape<hf::dualn<8>> {
-  static constexpr int TW = 8, TH = 4, NT = 128;
 };
 #ifndef HF_F32_TILE_W
 #define HF_F32_TILE_W 32
@@ -377,9 +381,9 @@ template <> struct FldOf<float> {
 };
 template <int N> struct FldOf<hf::dualn<N>> {
   static Fld<hf::dualn<N>> r(const void* v, const void* d) {
-    return Fld<hf::dualn<N>>{(const double*)v, (const double*)d};
+    return Fld<hf::dualn<N>>{(const double*)v, (const double*)d, N};
   }
-  static FldW<hf::dualn<N>> w(void* v, void* d) { return FldW<hf::dualn<N>>{(double*)v, (double*)d}; }
+  static FldW<hf::dualn<N>> w(void* v, void* d) { return FldW<hf::dualn<N>>{(double*)v, (double*)d, N, 1}; }
 };
 template <class T> Fld<T> fld(const void* v, const void* d) { return FldOf<T>::r(v, d); }
 template <class T> FldW<T> fldw(void* v, void* d) { return FldOf<T>::w(v, d); }

@@ -58,14 +58,18 @@ template <> struct Fld<dual> {
   const double* d;  // may be null: tangent 0 (constant bed)
   HF_HD dual ld(int64_t k) const { return dual{v[k], d ? d[k] : 0.0}; }
 };
-// DualN<N>: value plane, tangents per cell (d[cell * N + k]; null: 0).
+// DualN<N>: value plane, tangents of a cell at d[cell * ds + k] (null: 0).
+// ds == N for a whole DualN<N> field; a view of N components out of a
+// wider field (ds > N, d offset to the first component) is how the stage
+// kernels of DualN<M>, M > N, run over one group of components at a time.
 template <int N> struct Fld<dualn<N>> {
   const double* v;
   const double* d;
+  int ds;
   HF_HD dualn<N> ld(int64_t k) const {
     dualn<N> r;
     r.v = v[k];
-    for (int i = 0; i < N; ++i) r.d[i] = d ? d[k * N + i] : 0.0;
+    for (int i = 0; i < N; ++i) r.d[i] = d ? d[k * ds + i] : 0.0;
     return r;
   }
 };
@@ -90,12 +94,16 @@ template <> struct FldW<dual> {
     d[k] = x.d;
   }
 };
+// wv == 0: the view stores tangents only (values are stored by the view of
+// the last component group, after every group has read them).
 template <int N> struct FldW<dualn<N>> {
   double* v;
   double* d;
+  int ds;
+  int wv;
   HF_HD void st(int64_t k, const dualn<N>& x) const {
-    v[k] = x.v;
-    for (int i = 0; i < N; ++i) d[k * N + i] = x.d[i];
+    if (wv) v[k] = x.v;
+    for (int i = 0; i < N; ++i) d[k * ds + i] = x.d[i];
   }
 };
 
@@ -166,13 +174,11 @@ struct PointSrc {  // source.rs:36-44
   double q;
 };
 
-template <class T> struct StageArgs {
+// Launch arguments of a stage: the members that do not depend on the
+// scalar type, then (StageArgs<T>) the field views.
+struct StageCommon {
   int64_t nr, nc;
   double dx, dy;
-  Fld<T> ih, ihu, ihv;  // stencil input
-  FldW<T> oh, ohu, ohv;  // output
-  Fld<T> nh, nhu, nhv;  // U^n (final SSP-RK2 stage)
-  Fld<T> z, man;
   int bc_kind[4];  // N, S, W, E
   double bc_param[4];
   int dt_mode;  // 0: dt_host; 1: from the device clock
@@ -203,6 +209,13 @@ template <class T> struct StageArgs {
   const float* zdev;
   int datum_log2;
   int64_t datum_cols;
+};
+
+template <class T> struct StageArgs : StageCommon {
+  Fld<T> ih, ihu, ihv;  // stencil input
+  FldW<T> oh, ohu, ohv;  // output
+  Fld<T> nh, nhu, nhv;  // U^n (final SSP-RK2 stage)
+  Fld<T> z, man;
 };
 
 // Bed of one cell in the tile's frame. double/dual: the stored bed. float:

@@ -43,8 +43,10 @@ enum hf2d_status {
  * FP32 precision form (no reference counterpart: FP32 state and flux
  * arithmetic, bed as an FP64 datum per 64x64 block + FP32 deviation, FP64
  * dt reduction and Kahan-compensated clock), or the vector-mode DualN<N>
- * (autograd/src/dual_n.rs, Real impl real.rs:161-195) for N = 2, 4, 8
- * derivatives in one pass, with the reference's DualN arithmetic.
+ * (autograd/src/dual_n.rs, Real impl real.rs:161-195) for N = 2, 4, 8, 16
+ * derivatives in one pass, with the reference's DualN arithmetic (the N
+ * the reference's vector-mode example instantiates,
+ * solver-2d/examples/m1_forward_scaling.rs:184-187).
  *
  * Tangent arrays at the boundary: Dual, one per value (same layout as the
  * values); DualN<N>, N per value, innermost: the state tangents are
@@ -56,7 +58,8 @@ enum hf2d_scalar_kind {
   HF2D_F32 = 2,
   HF2D_DUAL2 = 3,
   HF2D_DUAL4 = 4,
-  HF2D_DUAL8 = 5
+  HF2D_DUAL8 = 5,
+  HF2D_DUAL16 = 6
 };
 
 /* Boundary kinds (boundary.rs:80-106). Sides are ordered N, S, W, E

@@ -204,6 +204,7 @@ template <> inline DualN<2> from_f64<DualN<2>>(double x) { return dn_constant<2>
 template <> inline DualN<3> from_f64<DualN<3>>(double x) { return dn_constant<3>(x); }
 template <> inline DualN<4> from_f64<DualN<4>>(double x) { return dn_constant<4>(x); }
 template <> inline DualN<8> from_f64<DualN<8>>(double x) { return dn_constant<8>(x); }
+template <> inline DualN<16> from_f64<DualN<16>>(double x) { return dn_constant<16>(x); }
 
 template <class T> inline T zero() { return from_f64<T>(0.0); }
 template <class T> inline T one() { return from_f64<T>(1.0); }

@@ -16,7 +16,7 @@
 using namespace hfo;
 
 namespace {
-// kind: 0 f64, 1 Dual, N (2, 4, 8) DualN<N>
+// kind: 0 f64, 1 Dual, N (2, 4, 8, 16) DualN<N>
 struct Handle {
   int kind = 0;
   Sim<double> f;
@@ -24,6 +24,7 @@ struct Handle {
   Sim<DualN<2>> d2;
   Sim<DualN<4>> d4;
   Sim<DualN<8>> d8;
+  Sim<DualN<16>> d16;
 };
 
 template <class F> auto visit(Handle* h, F&& fn) {
@@ -32,6 +33,7 @@ template <class F> auto visit(Handle* h, F&& fn) {
     case 2: return fn(h->d2);
     case 4: return fn(h->d4);
     case 8: return fn(h->d8);
+    case 16: return fn(h->d16);
     default: return fn(h->f);
   }
 }

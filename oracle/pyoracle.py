@@ -98,14 +98,14 @@ def dualn_probe(a, b, c):
 
 class OracleSim:
     """One grid on the CPU restatement. States are (n_rows, n_cols, 3) float64;
-    Dual tangents the same shape; DualN<N> (tangents=N in 2, 4, 8) tangents
+    Dual tangents the same shape; DualN<N> (tangents=N in 2, 4, 8, 16) tangents
     (n_rows, n_cols, 3, N) for the state and (n_rows, n_cols, N) for bed and
     Manning."""
 
     def __init__(self, n_rows, n_cols, dx, dy, dual=False, tangents=None):
         self.n_rows, self.n_cols, self.dx, self.dy = int(n_rows), int(n_cols), float(dx), float(dy)
-        if tangents is not None and tangents not in (2, 4, 8):
-            raise ValueError("tangents must be 2, 4 or 8")
+        if tangents is not None and tangents not in (2, 4, 8, 16):
+            raise ValueError("tangents must be 2, 4, 8 or 16")
         self.n_tan = int(tangents) if tangents else (1 if dual else 0)
         self.dual = self.n_tan > 0
         self._h = lib().hfo_create(self.n_rows, self.n_cols, self.dx, self.dy, self.n_tan)

@@ -71,7 +71,7 @@ def _zonal_case(n_params, n=32, steps=60, dt=0.01, dual=None, seed_zone=None):
     return sim.get_state()
 
 
-@pytest.mark.parametrize("n_params", [2, 4, 8])
+@pytest.mark.parametrize("n_params", [2, 4, 8, 16])
 def test_dualn_tangents_equal_seeded_dual_runs_on_the_f64_value_path(n_params):
     """With the quotient value a / b (the f64 path), DualN values are the f64
     run's bit for bit and each tangent is the seeded Dual run's to rounding:
@@ -88,7 +88,7 @@ def test_dualn_tangents_equal_seeded_dual_runs_on_the_f64_value_path(n_params):
         set_dualn_div_mode(0)
 
 
-@pytest.mark.parametrize("n_params", [2, 4, 8])
+@pytest.mark.parametrize("n_params", [2, 4, 8, 16])
 def test_dualn_reference_value_path(n_params):
     """The reference's DualN quotient a * (1 / b) moves values by the last
     bit only; the tangents then follow the branches that value path takes

@@ -212,7 +212,7 @@ def test_graph_replays_interleaved_with_host_calls_bitwise(rain):
     assert g.time() == o.time() and np.array_equal(g.get_state(), o.get_state())
 
 
-@pytest.mark.parametrize("n", [2, 4, 8])
+@pytest.mark.parametrize("n", [2, 4, 8, 16])
 def test_dualn_bitwise(n):
     """Vector-mode DualN<N> (autograd/src/dual_n.rs) on the GPU: the fused
     driver with rain and point sources, then a host-side loop of
