@@ -42,6 +42,9 @@ struct hf2d_sim {
   double bc_param[4] = {0, 0, 0, 0};
   hf2d_config cfg{};
   bool ws_valid = false;  // ctrl->ws[parity] holds the current state's maxima
+  // enqueue_steps left driver steps in flight: once they finish, the slots
+  // hold the maxima of the state they leave (hf2d_synchronize records that)
+  bool driver_async = false;
   bool ws_with_bcs = false;
   // active-tile list (f64 and f32 kinds): per-tile wet flags of U^n, quiet
   // run lengths, pinned tiles (edge, point source), scratch, the list and
@@ -77,6 +80,14 @@ struct hf2d_sim {
 namespace {
 
 constexpr int64_t GRAPH_STEPS = 16;
+
+// The state, mesh, boundaries or configuration changed: the CFL slots and
+// the active-tile flags no longer describe it.
+void state_changed(hf2d_sim* s) {
+  s->ws_valid = false;
+  s->list_valid = false;
+  s->driver_async = false;
+}
 
 void drop_graph(hf2d_sim* s) {
   if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
@@ -318,8 +329,7 @@ template <class T> int host_dt_step(hf2d_sim* s, double dt, bool rk2) {
   } else {
     s->cur = oth;
   }
-  s->ws_valid = false;
-  s->list_valid = false;
+  state_changed(s);
   drop_graph(s);
   return HF2D_OK;
 }
@@ -525,10 +535,14 @@ template <class T> int run_driver(hf2d_sim* s, bool use_target, double t_target,
   rc = write_ctrl(s, &c);
   if (rc) return rc;
   s->ws_valid = false;  // the slots now follow the driver
+  s->driver_async = false;
   if (!use_target) {
     rc = enqueue_n<T>(s, n_steps);
     if (rc) return rc;
-    if (!poll) return HF2D_OK;
+    if (!poll) {
+      s->driver_async = true;
+      return HF2D_OK;
+    }
     rc = read_ctrl(s, &c);
     if (rc) return rc;
     rc = harvest_timing(s);
@@ -582,8 +596,7 @@ template <class T> int forcing_only(hf2d_sim* s, double dt, int friction, double
   a.wave_speed = 0;
   hipLaunchKernelGGL((tail_kernel<T>), dim3(tail_grid(s)), dim3(256), 0, s->stream, a, 0);
   HIPCHK(s, hipGetLastError());
-  s->ws_valid = false;
-  s->list_valid = false;
+  state_changed(s);
   drop_graph(s);
   return HF2D_OK;
 }
@@ -794,8 +807,7 @@ int hf2d_upload_mesh(hf2d_sim* s, const double* bed, const double* bed_tangent, 
     else
       HIPCHK(s, hipMemset(s->mant, 0, tb));
   }
-  s->ws_valid = false;
-  s->list_valid = false;
+  state_changed(s);
   drop_graph(s);
   return HF2D_OK;
 }
@@ -860,8 +872,7 @@ int hf2d_upload_state(hf2d_sim* s, const double* aos, const double* tan_aos) {
       if (rc) return rc;
     }
   }
-  s->ws_valid = false;
-  s->list_valid = false;
+  state_changed(s);
   drop_graph(s);
   return HF2D_OK;
 }
@@ -919,8 +930,7 @@ int hf2d_set_boundaries(hf2d_sim* s, const int32_t kinds[4], const double params
     s->bc_kind[k] = kinds[k];
     s->bc_param[k] = params ? params[k] : 0.0;
   }
-  s->ws_valid = false;
-  s->list_valid = false;
+  state_changed(s);
   drop_graph(s);
   return HF2D_OK;
 }
@@ -984,8 +994,7 @@ int hf2d_apply_point_sources(hf2d_sim* s, const hf2d_point_source* src, int64_t 
   s->srcs = saved;
   s->n_src = saved_n;
   s->cap_src = saved_cap;
-  s->ws_valid = false;
-  s->list_valid = false;
+  state_changed(s);
   drop_graph(s);
   return rc;
 }
@@ -1010,8 +1019,7 @@ int hf2d_configure(hf2d_sim* s, const hf2d_config* cfg, const hf2d_point_source*
   rc = upload_pins(s, src, k);
   if (rc) return rc;
   s->cfg = *cfg;
-  s->ws_valid = false;
-  s->list_valid = false;
+  state_changed(s);
   drop_graph(s);
   return HF2D_OK;
 }
@@ -1132,6 +1140,8 @@ int hf2d_synchronize(hf2d_sim* s) {
   if (rc) return rc;
   rc = harvest_timing(s);
   if (rc) return rc;
+  if (!s->driver_async) return HF2D_OK;  // nothing of the driver's to report
+  s->driver_async = false;
   s->ws_valid = (c.status == 0);
   s->ws_with_bcs = s->cfg.cfl_with_bcs != 0;
   return ctrl_error(s, c, c.t_target);

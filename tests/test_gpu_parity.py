@@ -245,6 +245,31 @@ def test_dualn_bitwise(n):
     assert np.array_equal(gv, ov) and np.array_equal(gt, ot, equal_nan=True)
 
 
+@pytest.mark.parametrize("between", ["synchronize", "enqueue_sync_set_state"])
+def test_synchronize_does_not_validate_stale_cfl_slots(between):
+    """hf2d_synchronize records the driver's CFL maxima only after enqueued
+    driver steps: after set_state, a synchronize must leave the next driver
+    call to recompute them (a stale, zeroed slot would give dt = the whole
+    remaining time)."""
+    _say(f"synchronize after state change: {between}")
+    c = cases.bumpy_wet_dry(40, 52, 5)
+    runs = []
+    for sync in (False, True):
+        g = c.load(_hf().Grid2D(c.nr, c.nc, c.dx, c.dy))
+        g.configure(cfl=0.4, max_dt=1e9)
+        if sync and between == "enqueue_sync_set_state":
+            g.enqueue_steps(3)
+            g.set_state(c.state)
+            g.set_time(0.0, 0)
+        if sync:
+            g.synchronize()
+        g.run_until(2.0)
+        runs.append((g.time(), g.get_state()))
+        g.close()
+    assert runs[0][0] == runs[1][0] and runs[0][0][1] > 5
+    assert np.array_equal(runs[0][1], runs[1][1])
+
+
 def test_forward_euler_driver_bitwise():
     _say("forward euler driver")
     c = cases.bumpy_wet_dry(40, 70, 5)
