@@ -297,6 +297,42 @@ def uk_ea(args):
     return out
 
 
+def huasco_day(args):
+    """The Huasco reach from its GeoTIFF inputs (200 x 67 cells at 30 m),
+    one simulated day on the GPU: the sealed, source-free f64 run of
+    huasco_closed_domain.rs --no-inflow (the reference's cross-validation
+    against SynxFlow) and the Dual run of huasco_calibrate_tree.rs (value
+    and d h / d n_tree in one pass). SynxFlow's time for its run of the same
+    day, on an RTX 3050 (docs/xval-synxflow-huasco-results.md:126), is given
+    for context."""
+    from hydroflux_amd import huasco as H
+    from hydroflux_amd.ascii_grid import read_ascii_grid
+
+    d = os.path.join(ROOT, "tests", "golden", "huasco")
+    inp = H.load_inputs(os.path.join(d, "huasco_subset_dem.tif"), os.path.join(d, "huasco_subset_landcover.tif"),
+                        os.path.join(d, "huasco_subset_acc.tif"))
+    syn, _ = read_ascii_grid(os.path.join(d, "synxflow_noflow_h_86400.asc.gz"))
+    out = {"grid": list(inp.mesh.bed.shape), "synxflow_gpu_s_context": 6.5}
+    for name, make in (("sealed_f64", lambda: H.closed_domain_grid(inp, no_inflow=True)[0]),
+                       ("calibration_dual", lambda: H.calibration_grid(inp, H.N_TREE_INIT))):
+        g = make()
+        g.synchronize()
+        t0 = time.perf_counter()
+        g.run_until(H.SECONDS_PER_DAY)
+        g.synchronize()
+        wall = time.perf_counter() - t0
+        steps = g.time()[1]
+        st = g.get_state()
+        h = (st[0] if isinstance(st, tuple) else st)[..., 0]
+        out[name] = {"steps": steps, "wall_s": wall, "us_per_step": wall / steps * 1e6,
+                     "cell_updates_per_s": h.size * steps / wall}
+        if name == "sealed_f64":
+            out[name]["vs_synxflow"] = H.xval_metrics(h, syn)
+        g.close()
+        log(f"[bench] Huasco {name}: {steps} steps in {wall:.2f} s ({wall / steps * 1e6:.1f} us/step)")
+    return out
+
+
 def _zonal_pass(n, steps, kind, n_params, seed_zone=None):
     """solver-2d/examples/m1_forward_scaling.rs on the GPU: dam break (h = 1
     left half, 0.1 right half), transmissive edges, Manning 0.03 in
@@ -474,8 +510,8 @@ def main():
     ap.add_argument("--size", type=int, default=4096)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="headline FP64 line only (and cpu_baseline)")
-    ap.add_argument("--sections", default="precision,c1,dry,ukea,forward,batch,serving",
-                    help="comma list of extra sections: precision, c1, dry, ukea, forward, batch, serving")
+    ap.add_argument("--sections", default="precision,c1,dry,ukea,huasco,forward,batch,serving",
+                    help="comma list of extra sections: precision, c1, dry, ukea, huasco, forward, batch, serving")
     ap.add_argument("--dry-warmup", type=int, default=500)
     ap.add_argument("--acc-steps", type=int, default=2000, help="FP64 steps for the FP32 accuracy check (0: skip)")
     ap.add_argument("--serve-requests", type=int, default=64)
@@ -547,6 +583,8 @@ def main():
         out["dry_domain"] = dry_domain(args)
     if "ukea" in sections:
         out["uk_ea"] = uk_ea(args)
+    if "huasco" in sections:
+        out["huasco"] = huasco_day(args)
     if "forward" in sections:
         out["forward_scaling"] = forward_scaling(args)
     if "batch" in sections or "serving" in sections:
