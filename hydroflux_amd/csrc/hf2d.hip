@@ -51,6 +51,7 @@ struct hf2d_sim {
   // its length (tile_list[n_tiles]).
   unsigned char *tile_wet = nullptr, *tile_run = nullptr, *tile_pin = nullptr, *tile_act = nullptr;
   int* tile_list = nullptr;
+  unsigned long long* ws_part = nullptr;  // CFL slots of the final stage (WS_SLOTS x WS_STRIDE)
   int tiles_x = 0, tiles_y = 0;
   bool list_valid = false;  // tile_wet / tile_run follow the current state
   // driver forcing series ([0] rain, [1] point sources) and gauges
@@ -378,6 +379,7 @@ template <class T> int enqueue_step(hf2d_sim* s) {
   b.rain_rate = s->cfg.rain_rate;
   b.n_src = s->n_src;
   b.wave_speed = 1;
+  b.ws_part = s->ws_part;
   const bool series = s->series_host[0].n > 0 || s->series_host[1].n > 0;
   b.use_series = series ? 1 : 0;
   if (list_usable(s)) {
@@ -433,7 +435,7 @@ template <class T> int enqueue_step(hf2d_sim* s) {
       s->ev_used += 2;
     }
   }
-  hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(1), 0, s->stream, s->ctrl);
+  hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(WS_SLOTS), 0, s->stream, s->ctrl, s->ws_part);
   HIPCHK(s, hipGetLastError());
   if (s->n_gauges > 0) {
     hipLaunchKernelGGL((gauge_kernel<T>), dim3(1), dim3(64), 0, s->stream, s->ctrl,
@@ -728,6 +730,8 @@ int hf2d_create(int64_t n_rows, int64_t n_cols, double dx, double dy, int32_t sc
       return cleanup(HF2D_HIP_ERROR);
     if (upload_pins(s, nullptr, 0) != HF2D_OK) return cleanup(HF2D_HIP_ERROR);
   }
+  if (!zalloc((void**)&s->ws_part, (size_t)WS_SLOTS * WS_STRIDE * sizeof(unsigned long long)))
+    return cleanup(HF2D_HIP_ERROR);
   if (hipMalloc(&s->ctrl, sizeof(Ctrl)) != hipSuccess) return cleanup(HF2D_HIP_ERROR);
   Ctrl c{};
   c.t_target = 0.0;
@@ -750,7 +754,7 @@ int hf2d_destroy(hf2d_sim* s) {
   void* bufs[] = {s->z,        s->zt,       s->man,      s->mant,     s->datum,    s->zdev,
                   s->ctrl,     s->srcs,     s->tile_wet, s->tile_run, s->tile_pin, s->tile_act,
                   s->tile_list, s->series_buf[0], s->series_buf[1], s->series_dev, s->gauge_cells,
-                  s->gauge_buf, s->gauge_count};
+                  s->gauge_buf, s->gauge_count, s->ws_part};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   for (hipEvent_t e : s->ev) (void)hipEventDestroy(e);

@@ -83,6 +83,14 @@ template <> struct Shape<float> {
 // Workgroup maxima of the CFL wave speeds into dst (64-bit atomic max of
 // the non-negative doubles' bit patterns). With wet_dst (then red has 3
 // rows), also whether any thread saw a wet output cell, one byte.
+// CFL maxima slots of the driver's final stage: 64-bit atomic maxima of one
+// address from every workgroup serialize (device-scope atomics are resolved
+// outside the per-XCD L2s), so the workgroups of a launch spread over
+// WS_SLOTS addresses, one 128-byte line apart, and finalize_kernel folds
+// them. Max is associative, so the result is the single-address one.
+constexpr int WS_SLOTS = 512;
+constexpr int WS_STRIDE = 16;  // u64 per slot
+
 __device__ inline void block_max_to(double sx, double sy, double (*red)[16], unsigned long long* dst,
                                     bool wet = false, unsigned char* wet_dst = nullptr) {
   const int tid = threadIdx.x;
@@ -131,6 +139,8 @@ __global__ __launch_bounds__(Shape<T>::NT) void stage_kernel(StageArgs<T> a) {
   const int64_t ty = tile / a.tiles_x, tx = tile % a.tiles_x;
   const int64_t gi0 = ty * TH, gj0 = tx * TW;
   const int tid = threadIdx.x;
+  typename St::Pre pre{};
+  if constexpr (St::kPrefetch) St::template prefetch<MODE>(a, gi0, gj0, tid, &pre);
   St::phase_load(s, a, gi0, gj0, tid, NT);
   __syncthreads();
   St::phase_diffs(s, a, gi0, gj0, tid, NT);
@@ -141,10 +151,12 @@ __global__ __launch_bounds__(Shape<T>::NT) void stage_kernel(StageArgs<T> a) {
   __syncthreads();
   double wsx = 0.0, wsy = 0.0;
   bool wet = false;
-  St::template phase_update<MODE>(s, a, gi0, gj0, tid, NT, dt, &wsx, &wsy, &wet);
+  St::template phase_update<MODE>(s, a, gi0, gj0, tid, NT, dt, &wsx, &wsy, &wet, St::kPrefetch ? &pre : nullptr);
   if (MODE == hf::MODE_FINAL_RK2 && a.wave_speed) {
     Ctrl* c = a.ctrl;
-    block_max_to(wsx, wsy, s.red, &c->ws[c->parity ^ 1][0], wet, a.tile_wet ? a.tile_wet + tile : nullptr);
+    unsigned long long* dst =
+        a.ws_part ? a.ws_part + (size_t)(blockIdx.x % WS_SLOTS) * WS_STRIDE : &c->ws[c->parity ^ 1][0];
+    block_max_to(wsx, wsy, s.red, dst, wet, a.tile_wet ? a.tile_wet + tile : nullptr);
   }
 }
 
@@ -287,7 +299,41 @@ template <class T> __global__ void point_source_kernel(StageArgs<T> a, double dt
   }
 }
 
-__global__ void finalize_kernel(Ctrl* c) {
+// End of a driver step, one workgroup of WS_SLOTS threads: fold the final
+// stage's CFL slots (part, nullable) into ctrl->ws[parity ^ 1] and clear
+// them, then advance the clock and flip the slot.
+__global__ __launch_bounds__(WS_SLOTS) void finalize_kernel(Ctrl* c, unsigned long long* part) {
+  __shared__ unsigned long long red[2][WS_SLOTS / 64];
+  if (part) {
+    const int t = threadIdx.x;
+    unsigned long long mx = 0, my = 0;
+    if (t < WS_SLOTS) {
+      mx = part[(size_t)t * WS_STRIDE];
+      my = part[(size_t)t * WS_STRIDE + 1];
+      part[(size_t)t * WS_STRIDE] = 0ULL;
+      part[(size_t)t * WS_STRIDE + 1] = 0ULL;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      const unsigned long long ox = __shfl_xor(mx, off), oy = __shfl_xor(my, off);
+      mx = ox > mx ? ox : mx;
+      my = oy > my ? oy : my;
+    }
+    if ((t & 63) == 0) {
+      red[0][t >> 6] = mx;
+      red[1][t >> 6] = my;
+    }
+    __syncthreads();
+    if (t == 0) {
+      for (int k = 1; k < (int)(blockDim.x >> 6); ++k) {
+        mx = red[0][k] > mx ? red[0][k] : mx;
+        my = red[1][k] > my ? red[1][k] : my;
+      }
+      unsigned long long* w = &c->ws[c->parity ^ 1][0];
+      w[0] = mx > w[0] ? mx : w[0];
+      w[1] = my > w[1] ? my : w[1];
+    }
+  }
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   if (c->status != 0 || c->done != 0) return;
   if (c->compensated) {  // FP32 precision form: Kahan-compensated clock
@@ -416,6 +462,8 @@ __global__ __launch_bounds__(NT) void batch_stage_kernel(const StageArgs<T>* __r
     double dt;
     if (!hf::stage_prologue(a, it.tile == 0 && tid == 0, &dt)) continue;
     const int64_t gi0 = (int64_t)(it.tile / a.tiles_x) * TH, gj0 = (int64_t)(it.tile % a.tiles_x) * TW;
+    typename St::Pre pre{};
+    if constexpr (St::kPrefetch) St::template prefetch<MODE>(a, gi0, gj0, tid, &pre);
     St::phase_load(s, a, gi0, gj0, tid, NT);
     __syncthreads();
     St::phase_diffs(s, a, gi0, gj0, tid, NT);
@@ -426,7 +474,7 @@ __global__ __launch_bounds__(NT) void batch_stage_kernel(const StageArgs<T>* __r
     __syncthreads();
     double wsx = 0.0, wsy = 0.0;
     bool wet = false;
-    St::template phase_update<MODE>(s, a, gi0, gj0, tid, NT, dt, &wsx, &wsy, &wet);
+    St::template phase_update<MODE>(s, a, gi0, gj0, tid, NT, dt, &wsx, &wsy, &wet, St::kPrefetch ? &pre : nullptr);
     if (MODE == hf::MODE_FINAL_RK2 && a.wave_speed) {
       Ctrl* c = a.ctrl;
       block_max_to(wsx, wsy, s.red, &c->ws[c->parity ^ 1][0]);

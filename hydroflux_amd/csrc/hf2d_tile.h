@@ -43,6 +43,10 @@ enum { BC_TRANSMISSIVE = 0, BC_WALL = 1, BC_DISCHARGE = 2, BC_DEPTH = 3 };
 enum { SIDE_N = 0, SIDE_S = 1, SIDE_W = 2, SIDE_E = 3 };
 enum { MODE_STAGE = 0, MODE_FINAL_RK2 = 1 };
 
+#ifndef HF_PREFETCH
+#define HF_PREFETCH 1
+#endif
+
 #ifndef HF_BOUNDS_CHECK
 #define HF_BOUNDS_CHECK(cond) ((void)0)
 #endif
@@ -193,6 +197,10 @@ struct StageCommon {
   int n_src;
   const PointSrc* srcs;
   int wave_speed;  // reduce the output's wave speeds into ctrl->ws[parity ^ 1]
+  // Final driver stage: workgroups reduce into slot blockIdx % WS_SLOTS of
+  // this buffer instead of into ctrl->ws; finalize_kernel folds the slots
+  // in (null: straight into ctrl->ws).
+  unsigned long long* ws_part;
   int tiles_x, tiles_y;
   // Compacted active-tile list (null: every tile). Block b of a launch
   // takes list position b of the first *n_work; tile_wet (final stage,
@@ -968,10 +976,10 @@ template <class T, int TW, int TH> struct Stage {
   // (source.rs:130-165), rain (source.rs:88-101), point sources
   // (source.rs:52-69).
   static HF_HD void forcing(const StageArgs<T>& a, const ForcingNow& fn, int64_t gi, int64_t gj, double dt, T* h,
-                            T* hu, T* hv) {
+                            T* hu, T* hv, const T* man_pre = nullptr) {
     const T Z = cst<T>(0.0);
     if (a.friction && val(*h) > R(a.friction_dry_tol)) {
-      T n = a.man.ld(gi * a.nc + gj);
+      T n = man_pre ? *man_pre : a.man.ld(gi * a.nc + gj);
       if (val(n) != R(0)) {
         const R dt_g = R(dt * GRAVITY);
         T speed = div(rsqrt_(*hu * *hu + *hv * *hv), *h);
@@ -1007,13 +1015,40 @@ template <class T, int TW, int TH> struct Stage {
     }
   }
 
+  // The global values phase_update reads for a thread's first cell
+  // (k = tid), loaded before the first barrier so that their latency
+  // overlaps the LDS phases instead of following them. Scalar kinds only:
+  // a Dual or DualN copy would hold too many registers across the phases.
+  static constexpr bool kPrefetch =
+      HF_PREFETCH && (std::is_same<T, double>::value || std::is_same<T, float>::value);
+  struct Pre {
+    T hu, hv;        // stencil-input momentum (used where h_old > H_VEL)
+    T ph, phu, phv;  // U^n (final SSP-RK2 stage)
+    T man;           // Manning n (final stage: friction, Discharge ghosts)
+  };
+  template <int MODE>
+  static HF_HD void prefetch(const StageArgs<T>& a, int64_t gi0, int64_t gj0, int tid, Pre* p) {
+    const int64_t gi = gi0 + tid / TW, gj = gj0 + tid % TW;
+    if (tid >= TW * TH || gi >= a.nr || gj >= a.nc) return;
+    const int64_t idx = gi * a.nc + gj;
+    p->hu = a.ihu.ld(idx);
+    p->hv = a.ihv.ld(idx);
+    if (MODE == MODE_FINAL_RK2) {
+      p->ph = a.nh.ld(idx);
+      p->phu = a.nhu.ld(idx);
+      p->phv = a.nhv.ld(idx);
+      if (a.friction || a.cfl_with_bcs) p->man = a.man.ld(idx);
+    }
+  }
+
   // Phase 5: FV update with the explicit bed-slope source
   // (update.rs:1071-1210), then the stage epilogue: write U^(1), or the
   // SSP-RK2 combine (update.rs:1272-1286) + forcing + CFL wave speeds;
-  // *wet is set when an output cell is wet (not h <= H_DRY).
+  // *wet is set when an output cell is wet (not h <= H_DRY). pre: the
+  // prefetched values of cell k = tid (null: read them here).
   template <int MODE>
   static HF_HD void phase_update(S& s, const StageArgs<T>& a, int64_t gi0, int64_t gj0, int tid, int nt,
-                                 double dt, double* wsx, double* wsy, bool* wet) {
+                                 double dt, double* wsx, double* wsy, bool* wet, const Pre* pre = nullptr) {
     const T Z = cst<T>(0.0);
     const R dt_dx = R(dt / a.dx), dt_dy = R(dt / a.dy), rdt = R(dt);
     ForcingNow fnow{};
@@ -1023,6 +1058,7 @@ template <class T, int TW, int TH> struct Stage {
       const int64_t gi = gi0 + li, gj = gj0 + lj;
       if (gi >= a.nr || gj >= a.nc) continue;
       const int64_t idx = gi * a.nc + gj;
+      const Pre* pk = (pre && k == tid) ? pre : nullptr;
       const int c3 = S::i3(li, lj);
       const T h_old = s.h[c3];
       T nh, nhu, nhv;
@@ -1107,8 +1143,8 @@ template <class T, int TW, int TH> struct Stage {
         } else {
           T hu_old = Z, hv_old = Z;
           if (val(h_old) > R(H_VEL)) {
-            hu_old = a.ihu.ld(idx);
-            hv_old = a.ihv.ld(idx);
+            hu_old = pk ? pk->hu : a.ihu.ld(idx);
+            hv_old = pk ? pk->hv : a.ihv.ld(idx);
           }
           nh = new_h;
           nhu = hu_old - dhu + s_hu * rdt;
@@ -1116,7 +1152,7 @@ template <class T, int TW, int TH> struct Stage {
         }
       }
       if (MODE == MODE_FINAL_RK2) {
-        T ph = a.nh.ld(idx), phu = a.nhu.ld(idx), phv = a.nhv.ld(idx);
+        T ph = pk ? pk->ph : a.nh.ld(idx), phu = pk ? pk->phu : a.nhu.ld(idx), phv = pk ? pk->phv : a.nhv.ld(idx);
         nh = (ph + nh) * R(0.5);
         nhu = (phu + nhu) * R(0.5);
         nhv = (phv + nhv) * R(0.5);
@@ -1125,7 +1161,7 @@ template <class T, int TW, int TH> struct Stage {
           nhu = Z;
           nhv = Z;
         }
-        forcing(a, fnow, gi, gj, dt, &nh, &nhu, &nhv);
+        forcing(a, fnow, gi, gj, dt, &nh, &nhu, &nhv, pk ? &pk->man : nullptr);
         if (!(val(nh) <= R(H_DRY))) *wet = true;
         if (a.wave_speed) {
           T zn4[4];
@@ -1133,7 +1169,7 @@ template <class T, int TW, int TH> struct Stage {
           zn4[SIDE_S] = (a.nr >= 2 && gi == a.nr - 1) ? s.z[S::i3(li - 1, lj)] : s.z[c3];
           zn4[SIDE_W] = (a.nc >= 2 && gj == 0) ? s.z[S::i3(li, lj + 1)] : s.z[c3];
           zn4[SIDE_E] = (a.nc >= 2 && gj == a.nc - 1) ? s.z[S::i3(li, lj - 1)] : s.z[c3];
-          T man = a.cfl_with_bcs ? a.man.ld(idx) : Z;
+          T man = a.cfl_with_bcs ? (pk ? pk->man : a.man.ld(idx)) : Z;
           cell_wave_speed(a, gi, gj, nh, nhu, nhv, s.z[c3], zn4, &man, wsx, wsy);
         }
       }

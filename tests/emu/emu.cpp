@@ -111,6 +111,13 @@ void launch(const StageArgs<T>& a, int mode, unsigned long long* dst) {
   for (int64_t b = 0; b < nb; ++b) {
     const int64_t tile = a.tile_list ? a.tile_list[b] : b;
     const int64_t gi0 = (tile / a.tiles_x) * TH, gj0 = (tile % a.tiles_x) * TW;
+    std::vector<typename St::Pre> pre(NT);
+    for (int t = 0; t < NT; ++t) {
+      if (mode == MODE_STAGE)
+        St::template prefetch<MODE_STAGE>(a, gi0, gj0, t, &pre[t]);
+      else
+        St::template prefetch<MODE_FINAL_RK2>(a, gi0, gj0, t, &pre[t]);
+    }
     for (int t = 0; t < NT; ++t) St::phase_load(*s, a, gi0, gj0, t, NT);
     for (int t = 0; t < NT; ++t) St::phase_diffs(*s, a, gi0, gj0, t, NT);
     for (int t = 0; t < NT; ++t) St::phase_faces(*s, a, gi0, gj0, t, NT);
@@ -119,9 +126,9 @@ void launch(const StageArgs<T>& a, int mode, unsigned long long* dst) {
     for (int t = 0; t < NT; ++t) {
       double wx = 0.0, wy = 0.0;
       if (mode == MODE_STAGE)
-        St::template phase_update<MODE_STAGE>(*s, a, gi0, gj0, t, NT, dt, &wx, &wy, &wet);
+        St::template phase_update<MODE_STAGE>(*s, a, gi0, gj0, t, NT, dt, &wx, &wy, &wet, &pre[t]);
       else
-        St::template phase_update<MODE_FINAL_RK2>(*s, a, gi0, gj0, t, NT, dt, &wx, &wy, &wet);
+        St::template phase_update<MODE_FINAL_RK2>(*s, a, gi0, gj0, t, NT, dt, &wx, &wy, &wet, &pre[t]);
       gx = rmax(gx, wx);
       gy = rmax(gy, wy);
     }
