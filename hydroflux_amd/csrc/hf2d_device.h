@@ -23,7 +23,7 @@ template <class T> struct Shape;
 #define HF_F64_TILE_W 32
 #endif
 #ifndef HF_F64_TILE_H
-#define HF_F64_TILE_H 16
+#define HF_F64_TILE_H 24
 #endif
 #ifndef HF_F64_THREADS
 #define HF_F64_THREADS 1024

@@ -22,7 +22,7 @@ def _mode():
     set_pow_mode(0)
 
 
-TILES = [(32, 16), (8, 4), (4, 8)]
+TILES = [(32, 24), (32, 16), (8, 4), (4, 8)]
 CASES = [
     (cases.stoker(100), None),
     (cases.thacker(40), None),
@@ -99,7 +99,7 @@ def test_driver_errors_match_oracle():
         assert s.time()[1] == 3
 
 
-@pytest.mark.parametrize("tile", [(8, 4), (4, 8), (16, 8), (32, 16)], ids=lambda t: f"{t[0]}x{t[1]}")
+@pytest.mark.parametrize("tile", [(8, 4), (4, 8), (16, 8), (32, 16), (32, 24)], ids=lambda t: f"{t[0]}x{t[1]}")
 @pytest.mark.parametrize("west", [cases.DISCHARGE, cases.DEPTH], ids=["discharge", "depth"])
 def test_active_tile_list_bitwise(tile, west):
     """The compacted active-tile list leaves settled dry tiles out; results
@@ -126,7 +126,7 @@ def test_active_tile_list_bitwise(tile, west):
     o.set_state(st)
     for s in (o, on, off):
         s.run_steps(25)
-    if tile != (32, 16):
+    if tile[0] * tile[1] < 512:  # on this 60 x 84 grid, larger tiles all touch the edge or the water
         assert skipped > 0
     assert o.time() == on.time() == off.time()
     assert np.array_equal(on.get_state(), off.get_state())
