@@ -49,9 +49,24 @@ __device__ inline unsigned tile_of_block(unsigned b, unsigned nb) {
 template <> struct Shape<double> {
   static constexpr int TW = HF_F64_TILE_W, TH = HF_F64_TILE_H, NT = HF_F64_THREADS;
 };
+#ifndef HF_DUAL_TILE_W
+#define HF_DUAL_TILE_W 16
+#endif
+#ifndef HF_DUAL_TILE_H
+#define HF_DUAL_TILE_H 16
+#endif
+#ifndef HF_DUAL_THREADS
+#define HF_DUAL_THREADS 1024
+#endif
 template <> struct Shape<dual> {
-  static constexpr int TW = 16, TH = 16, NT = 256;
+  static constexpr int TW = HF_DUAL_TILE_W, TH = HF_DUAL_TILE_H, NT = HF_DUAL_THREADS;
 };
+#ifndef HF_DUAL2_THREADS
+#define HF_DUAL2_THREADS 512
+#endif
+#ifndef HF_DUAL4_THREADS
+#define HF_DUAL4_THREADS 512
+#endif
 // DualN<N>: (N + 1) doubles per LDS value; tiles sized to stay within one
 // workgroup's LDS. For N > 4 the stage kernels run as N / 4 launches of the
 // DualN<4> kernel, one per group of 4 tangent components (a DualN<8> tile
@@ -59,14 +74,17 @@ template <> struct Shape<dual> {
 // cells loaded per cell updated), so its tiles are DualN<4>'s.
 template <int N> struct Shape<hf::dualn<N>> {
   static_assert(N > 4 && N % 4 == 0, "DualN<N> wider than 4 runs in groups of 4 components");
-  static constexpr int TW = 8, TH = 8, NT = 256;
+  static constexpr int TW = 8, TH = 8, NT = HF_DUAL4_THREADS;
 };
 template <> struct Shape<hf::dualn<2>> {
-  static constexpr int TW = 16, TH = 8, NT = 256;
+  static constexpr int TW = 16, TH = 8, NT = HF_DUAL2_THREADS;
 };
 template <> struct Shape<hf::dualn<4>> {
-  static constexpr int TW = 8, TH = 8, NT = 256;
+  static constexpr int TW = 8, TH = 8, NT = HF_DUAL4_THREADS;
 };
+// FP32 tiles lie inside one 64 x 64 bed-datum block (shapes that divide
+// 64), so its results are the same for all of them; a 64 x 24 tile is 3 %
+// faster but straddles blocks and rounds the bed differently.
 #ifndef HF_F32_TILE_W
 #define HF_F32_TILE_W 32
 #endif

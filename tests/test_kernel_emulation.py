@@ -22,7 +22,7 @@ def _mode():
     set_pow_mode(0)
 
 
-TILES = [(32, 24), (32, 16), (8, 4), (4, 8)]
+TILES = [(64, 24), (32, 24), (32, 16), (8, 4), (4, 8)]
 CASES = [
     (cases.stoker(100), None),
     (cases.thacker(40), None),
