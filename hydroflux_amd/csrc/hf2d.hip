@@ -52,6 +52,8 @@ struct hf2d_sim {
   unsigned char *tile_wet = nullptr, *tile_run = nullptr, *tile_pin = nullptr, *tile_act = nullptr;
   int* tile_list = nullptr;
   unsigned long long* ws_part = nullptr;  // CFL slots of the final stage (WS_SLOTS x WS_STRIDE)
+  int tw = 0, th = 0;        // stage tile shape: Shape<T>, or ShapeSmall<T> for a small grid
+  bool small_tiles = false;
   int tiles_x = 0, tiles_y = 0;
   bool list_valid = false;  // tile_wet / tile_run follow the current state
   // driver forcing series ([0] rain, [1] point sources) and gauges
@@ -166,9 +168,8 @@ template <class T> StageArgs<T> make_args(hf2d_sim* s, int in, int out, int un) 
   a.max_dt = s->cfg.max_dt;
   a.cfl_with_bcs = s->cfg.cfl_with_bcs;
   a.srcs = s->srcs;
-  constexpr int TW = Shape<T>::TW, TH = Shape<T>::TH;
-  a.tiles_x = (int)((s->nc + TW - 1) / TW);
-  a.tiles_y = (int)((s->nr + TH - 1) / TH);
+  a.tiles_x = (int)((s->nc + s->tw - 1) / s->tw);
+  a.tiles_y = (int)((s->nr + s->th - 1) / s->th);
   return a;
 }
 
@@ -212,14 +213,36 @@ template <int N> StageArgs<hf::dualn<4>> group_args(const StageArgs<hf::dualn<N>
   return b;
 }
 
-template <class T> int launch_stage_kernel(hf2d_sim* s, const StageArgs<T>& a, int mode) {
+template <class T, class Sh> int launch_shaped(hf2d_sim* s, const StageArgs<T>& a, int mode) {
   const unsigned grid = (unsigned)((int64_t)a.tiles_x * a.tiles_y);
   if (mode == hf::MODE_STAGE)
-    hipLaunchKernelGGL((stage_kernel<T, hf::MODE_STAGE>), dim3(grid), dim3(Shape<T>::NT), 0, s->stream, a);
+    hipLaunchKernelGGL((stage_kernel<T, Sh::TW, Sh::TH, Sh::NT, hf::MODE_STAGE>), dim3(grid), dim3(Sh::NT), 0,
+                       s->stream, a);
   else
-    hipLaunchKernelGGL((stage_kernel<T, hf::MODE_FINAL_RK2>), dim3(grid), dim3(Shape<T>::NT), 0, s->stream,
-                       a);
+    hipLaunchKernelGGL((stage_kernel<T, Sh::TW, Sh::TH, Sh::NT, hf::MODE_FINAL_RK2>), dim3(grid), dim3(Sh::NT), 0,
+                       s->stream, a);
   HIPCHK(s, hipGetLastError());
+  return HF2D_OK;
+}
+
+template <class T> int launch_stage_kernel(hf2d_sim* s, const StageArgs<T>& a, int mode) {
+  if constexpr (ShapeSmall<T>::on) {
+    if (s->small_tiles) return launch_shaped<T, ShapeSmall<T>>(s, a, mode);
+  }
+  return launch_shaped<T, Shape<T>>(s, a, mode);
+}
+
+// The grid's stage tile shape, fixed at creation.
+template <class T> int choose_tiles(hf2d_sim* s) {
+  const long long tiles = ((s->nr + Shape<T>::TH - 1) / Shape<T>::TH) * ((s->nc + Shape<T>::TW - 1) / Shape<T>::TW);
+  if constexpr (ShapeSmall<T>::on) {
+    s->small_tiles = tiles < SMALL_GRID_TILES;
+    s->tw = s->small_tiles ? ShapeSmall<T>::TW : Shape<T>::TW;
+    s->th = s->small_tiles ? ShapeSmall<T>::TH : Shape<T>::TH;
+  } else {
+    s->tw = Shape<T>::TW;
+    s->th = Shape<T>::TH;
+  }
   return HF2D_OK;
 }
 
@@ -344,7 +367,7 @@ template <class T> int enqueue_list_init(hf2d_sim* s) {
     hipLaunchKernelGGL(fill_bytes_kernel, dim3(g), dim3(256), 0, s->stream, s->tile_wet, nt, (unsigned char)0);
     hipLaunchKernelGGL(fill_bytes_kernel, dim3(g), dim3(256), 0, s->stream, s->tile_run, nt, (unsigned char)0);
     hipLaunchKernelGGL((tile_wet_init_kernel<T>), dim3(tail_grid(s)), dim3(256), 0, s->stream,
-                       fld<T>(s->st[s->cur][0], s->tg[s->cur][0]), s->nr, s->nc, Shape<T>::TW, Shape<T>::TH,
+                       fld<T>(s->st[s->cur][0], s->tg[s->cur][0]), s->nr, s->nc, s->tw, s->th,
                        s->tiles_x, s->tile_wet);
     HIPCHK(s, hipGetLastError());
     s->list_valid = true;
@@ -634,8 +657,7 @@ int upload_sources(hf2d_sim* s, const hf2d_point_source* src, int64_t k) {
 // holding a point source of the configured driver.
 int upload_pins(hf2d_sim* s, const hf2d_point_source* src, int64_t k) {
   if (!s->tile_pin) return HF2D_OK;
-  const int tw = s->kind == HF2D_F32 ? Shape<float>::TW : Shape<double>::TW;
-  const int th = s->kind == HF2D_F32 ? Shape<float>::TH : Shape<double>::TH;
+  const int tw = s->tw, th = s->th;
   std::vector<unsigned char> pin((size_t)s->tiles_x * s->tiles_y, 0);
   for (int ty = 0; ty < s->tiles_y; ++ty)
     for (int tx = 0; tx < s->tiles_x; ++tx)
@@ -689,6 +711,7 @@ int hf2d_create(int64_t n_rows, int64_t n_cols, double dx, double dy, int32_t sc
   s->kind = scalar_kind;
   s->n_tan = tangents_of(scalar_kind);
   s->cfg = default_config();
+  HF_DISPATCH(s, choose_tiles, s);
   const size_t n = (size_t)(n_rows * n_cols);
   const size_t bytes = n * elem_bytes(scalar_kind);
   const size_t tbytes = bytes * (size_t)s->n_tan;
@@ -719,10 +742,8 @@ int hf2d_create(int64_t n_rows, int64_t n_cols, double dx, double dy, int32_t sc
     if (!zalloc(&s->mant, tbytes)) return cleanup(HF2D_HIP_ERROR);
   }
   if (s->n_tan == 0) {
-    const int tw = scalar_kind == HF2D_F32 ? Shape<float>::TW : Shape<double>::TW;
-    const int th = scalar_kind == HF2D_F32 ? Shape<float>::TH : Shape<double>::TH;
-    s->tiles_x = (int)((n_cols + tw - 1) / tw);
-    s->tiles_y = (int)((n_rows + th - 1) / th);
+    s->tiles_x = (int)((n_cols + s->tw - 1) / s->tw);
+    s->tiles_y = (int)((n_rows + s->th - 1) / s->th);
     const size_t nt = (size_t)s->tiles_x * s->tiles_y;
     if (!zalloc((void**)&s->tile_wet, nt) || !zalloc((void**)&s->tile_run, nt) ||
         !zalloc((void**)&s->tile_pin, nt) || !zalloc((void**)&s->tile_act, nt) ||

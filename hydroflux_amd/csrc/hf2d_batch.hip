@@ -36,7 +36,7 @@ struct CfgInfo {
 // (16x16, 32x8, 64x4, 128x2) and the larger high-occupancy tiles.
 const CfgInfo CFG_F32[] = {{32, 32, 1024}, {32, 16, 512}, {16, 16, 256}, {32, 8, 256}, {64, 4, 256}, {128, 2, 256}};
 const CfgInfo CFG_F64[] = {{32, 24, 1024}, {32, 16, 1024}, {16, 16, 256}, {32, 8, 256}};
-const CfgInfo CFG_DUAL[] = {{16, 16, 1024}, {16, 16, 256}, {16, 8, 256}};
+const CfgInfo CFG_DUAL[] = {{16, 8, 512}, {16, 16, 1024}, {16, 8, 256}};
 
 template <class T> struct Cfgs;
 template <> struct Cfgs<float> {
@@ -113,15 +113,15 @@ template <> int occupancy_cfg<double>(int ci) {
 template <> int launch_cfg<dual>(int ci, hipStream_t st, const StageArgs<dual>* d1, const StageArgs<dual>* d2,
                                  const WorkItem* w, int n, int g) {
   switch (ci) {
-    case 0: return launch_pair<dual, 16, 16, 1024>(st, d1, d2, w, n, g);
-    case 1: return launch_pair<dual, 16, 16, 256>(st, d1, d2, w, n, g);
+    case 0: return launch_pair<dual, 16, 8, 512>(st, d1, d2, w, n, g);
+    case 1: return launch_pair<dual, 16, 16, 1024>(st, d1, d2, w, n, g);
     default: return launch_pair<dual, 16, 8, 256>(st, d1, d2, w, n, g);
   }
 }
 template <> int occupancy_cfg<dual>(int ci) {
   switch (ci) {
-    case 0: return occupancy_blocks<dual, 16, 16, 1024>();
-    case 1: return occupancy_blocks<dual, 16, 16, 256>();
+    case 0: return occupancy_blocks<dual, 16, 8, 512>();
+    case 1: return occupancy_blocks<dual, 16, 16, 1024>();
     default: return occupancy_blocks<dual, 16, 8, 256>();
   }
 }

@@ -32,6 +32,25 @@ template <class T> struct Shape;
 #define HF_XCD_REMAP 1
 #endif
 
+// Tiles for small grids. With fewer tiles than CUs, a step lasts about one
+// tile's phases, latency-bound, and a smaller tile is faster (16 x 8 with
+// 512 threads: UK EA Test 4, 400 x 200 cells, 40 us/step against 59 with
+// 32 x 24). A grid uses them when the default shape would give it fewer
+// than SMALL_GRID_TILES tiles (FP64 on the flood workload: the small tile
+// wins at 88 tiles, the default at 192 and more).
+template <class T> struct ShapeSmall {
+  static constexpr bool on = false;
+};
+template <> struct ShapeSmall<double> {
+  static constexpr bool on = true;
+  static constexpr int TW = 16, TH = 8, NT = 512;
+};
+template <> struct ShapeSmall<float> {
+  static constexpr bool on = true;
+  static constexpr int TW = 16, TH = 8, NT = 512;
+};
+constexpr long long SMALL_GRID_TILES = 160;
+
 // Workgroups are dispatched round-robin over the 8 XCDs (workgroup b lands on
 // XCD b % 8), each with its own L2. Renumber tiles so every XCD walks one
 // contiguous band of tiles: horizontally adjacent tiles, which share halo
@@ -53,10 +72,10 @@ template <> struct Shape<double> {
 #define HF_DUAL_TILE_W 16
 #endif
 #ifndef HF_DUAL_TILE_H
-#define HF_DUAL_TILE_H 16
+#define HF_DUAL_TILE_H 8
 #endif
 #ifndef HF_DUAL_THREADS
-#define HF_DUAL_THREADS 1024
+#define HF_DUAL_THREADS 512
 #endif
 template <> struct Shape<dual> {
   static constexpr int TW = HF_DUAL_TILE_W, TH = HF_DUAL_TILE_H, NT = HF_DUAL_THREADS;
@@ -139,9 +158,9 @@ __device__ inline void block_max_to(double sx, double sy, double (*red)[16], uns
   }
 }
 
-template <class T, int MODE>
-__global__ __launch_bounds__(Shape<T>::NT) void stage_kernel(StageArgs<T> a) {
-  constexpr int TW = Shape<T>::TW, TH = Shape<T>::TH, NT = Shape<T>::NT;
+// One stage of one TW x TH tile per workgroup of NT threads.
+template <class T, int TW, int TH, int NT, int MODE>
+__global__ __launch_bounds__(NT) void stage_kernel(StageArgs<T> a) {
   using St = hf::Stage<T, TW, TH>;
   __shared__ typename St::S s;
   double dt;

@@ -102,11 +102,15 @@ def test_batch_f32_alone_equals_in_mix_and_across_tile_configs():
         assert np.array_equal(other[0], alone[0]), cfg
 
 
-def test_batch_f32_matches_single_grid_f32():
-    _say("batch f32 vs single-grid f32")
+@pytest.mark.parametrize("lo,hi", [(200, 300), (470, 520)])
+def test_batch_f32_matches_single_grid_f32(lo, hi):
+    """The single grid takes 16 x 8 tiles below 160 tiles of 32 x 32 and
+    32 x 32 above; the batch runs 32 x 32. Tiles that divide the 64 x 64
+    bed-datum blocks give the FP32 form's bits whatever their shape."""
+    _say(f"batch f32 vs single-grid f32 {lo}-{hi}")
     from hydroflux_amd import Grid2D
 
-    sc = _mix(1, 5, 200, 300)[0]
+    sc = _mix(1, 5, lo, hi)[0]
     out, t, steps, _ = _run_batch("f32", [_spec(sc, 25.0)])[0]
     g = Grid2D(sc.n_rows, sc.n_cols, sc.dx, sc.dx, kind="f32")
     g.set_mesh(sc.bed, sc.manning)

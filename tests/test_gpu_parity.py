@@ -245,6 +245,28 @@ def test_dualn_bitwise(n):
     assert np.array_equal(gv, ov) and np.array_equal(gt, ot, equal_nan=True)
 
 
+@pytest.mark.parametrize("kind,nr,nc", [("f64", 520, 600), ("f64", 150, 100)])
+def test_default_and_small_grid_tiles_bitwise(kind, nr, nc):
+    """A grid takes the default stage tiles (f64 32 x 24) when they number
+    at least 160 and 16 x 8 tiles otherwise: 520 x 600 (22 x 19 = 418
+    default tiles) runs the default kernels, 150 x 100 the small ones, each
+    bit for bit with the oracle through the fused driver with friction,
+    rain and point sources and the host loop."""
+    _say(f"large-tile kernels {kind} {nr}x{nc}")
+    c = cases.bumpy_wet_dry(nr, nc, 3)
+    kw = {}
+    g = c.load(_hf().Grid2D(c.nr, c.nc, c.dx, c.dy, kind=kind), **kw)
+    o = c.load(OracleSim(c.nr, c.nc, c.dx, c.dy, dual=kind == "dual"), **kw)
+    for s in (g, o):
+        s.configure(cfl=0.4, max_dt=0.05, rain_rate=3e-4, sources=[(nr // 2, 7, 0.3), (10, nc - 9, -0.01)])
+        s.run_steps(6)
+        dt = s.cfl_time_step(0.4)
+        s.ssprk2_step(dt)
+        s.manning_friction_step(dt, 1e-6)
+    assert g.time() == o.time()
+    assert np.array_equal(g.get_state(), o.get_state())
+
+
 @pytest.mark.parametrize("between", ["synchronize", "enqueue_sync_set_state"])
 def test_synchronize_does_not_validate_stale_cfl_slots(between):
     """hf2d_synchronize records the driver's CFL maxima only after enqueued
