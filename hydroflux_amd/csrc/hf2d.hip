@@ -52,8 +52,8 @@ struct hf2d_sim {
   unsigned char *tile_wet = nullptr, *tile_run = nullptr, *tile_pin = nullptr, *tile_act = nullptr;
   int* tile_list = nullptr;
   unsigned long long* ws_part = nullptr;  // CFL slots of the final stage (WS_SLOTS x WS_STRIDE)
-  int tw = 0, th = 0;        // stage tile shape: Shape<T>, or ShapeSmall<T> for a small grid
-  bool small_tiles = false;
+  int tw = 0, th = 0;  // stage tile shape: Shape<T>, ShapeMid<T> or ShapeSmall<T> (by grid size)
+  int shape = 0;       // 0 Shape, 1 ShapeMid, 2 ShapeSmall
   int tiles_x = 0, tiles_y = 0;
   bool list_valid = false;  // tile_wet / tile_run follow the current state
   // driver forcing series ([0] rain, [1] point sources) and gauges
@@ -226,22 +226,35 @@ template <class T, class Sh> int launch_shaped(hf2d_sim* s, const StageArgs<T>& 
 }
 
 template <class T> int launch_stage_kernel(hf2d_sim* s, const StageArgs<T>& a, int mode) {
+  if constexpr (ShapeMid<T>::on) {
+    if (s->shape == 1) return launch_shaped<T, ShapeMid<T>>(s, a, mode);
+  }
   if constexpr (ShapeSmall<T>::on) {
-    if (s->small_tiles) return launch_shaped<T, ShapeSmall<T>>(s, a, mode);
+    if (s->shape == 2) return launch_shaped<T, ShapeSmall<T>>(s, a, mode);
   }
   return launch_shaped<T, Shape<T>>(s, a, mode);
 }
 
+long long tiles_of(const hf2d_sim* s, int tw, int th) { return ((s->nr + th - 1) / th) * ((s->nc + tw - 1) / tw); }
+
 // The grid's stage tile shape, fixed at creation.
 template <class T> int choose_tiles(hf2d_sim* s) {
-  const long long tiles = ((s->nr + Shape<T>::TH - 1) / Shape<T>::TH) * ((s->nc + Shape<T>::TW - 1) / Shape<T>::TW);
+  s->shape = 0;
+  s->tw = Shape<T>::TW;
+  s->th = Shape<T>::TH;
+  if (tiles_of(s, Shape<T>::TW, Shape<T>::TH) >= SMALL_GRID_TILES) return HF2D_OK;
+  if constexpr (ShapeMid<T>::on) {
+    if (tiles_of(s, ShapeMid<T>::TW, ShapeMid<T>::TH) >= MID_GRID_TILES) {
+      s->shape = 1;
+      s->tw = ShapeMid<T>::TW;
+      s->th = ShapeMid<T>::TH;
+      return HF2D_OK;
+    }
+  }
   if constexpr (ShapeSmall<T>::on) {
-    s->small_tiles = tiles < SMALL_GRID_TILES;
-    s->tw = s->small_tiles ? ShapeSmall<T>::TW : Shape<T>::TW;
-    s->th = s->small_tiles ? ShapeSmall<T>::TH : Shape<T>::TH;
-  } else {
-    s->tw = Shape<T>::TW;
-    s->th = Shape<T>::TH;
+    s->shape = 2;
+    s->tw = ShapeSmall<T>::TW;
+    s->th = ShapeSmall<T>::TH;
   }
   return HF2D_OK;
 }

@@ -33,13 +33,22 @@ template <class T> struct Shape;
 #endif
 
 // Tiles for small grids. With fewer tiles than CUs, a step lasts about one
-// tile's phases, latency-bound, and a smaller tile is faster (16 x 8 with
-// 512 threads: UK EA Test 4, 400 x 200 cells, 40 us/step against 59 with
-// 32 x 24). A grid uses them when the default shape would give it fewer
-// than SMALL_GRID_TILES tiles (FP64 on the flood workload: the small tile
-// wins at 88 tiles, the default at 192 and more).
+// tile's phases, latency-bound, and a smaller tile is faster. A grid takes
+// the default shape when it gives at least SMALL_GRID_TILES tiles (FP64 on
+// the flood workload: the default wins from 192 tiles on, 16 x 8 at 88);
+// else ShapeMid when that gives at least MID_GRID_TILES tiles; else
+// ShapeSmall. Measured FP64 us/step over full runs, 16x8x512 / 32x16x1024 /
+// 32x24x1024: UK EA Test 4 (400 x 200) 48.8 / 49.6 / 58.9, Test 8A
+// (199 x 481) 58.7 / 48.2 / 59.2, Huasco reach (200 x 67) 25.8 / 36.9 / 45.4.
 template <class T> struct ShapeSmall {
   static constexpr bool on = false;
+};
+template <class T> struct ShapeMid {
+  static constexpr bool on = false;
+};
+template <> struct ShapeMid<double> {
+  static constexpr bool on = true;
+  static constexpr int TW = 32, TH = 16, NT = 1024;
 };
 template <> struct ShapeSmall<double> {
   static constexpr bool on = true;
@@ -49,7 +58,11 @@ template <> struct ShapeSmall<float> {
   static constexpr bool on = true;
   static constexpr int TW = 16, TH = 8, NT = 512;
 };
-constexpr long long SMALL_GRID_TILES = 160;
+#ifndef HF_SMALL_GRID_TILES
+#define HF_SMALL_GRID_TILES 160
+#endif
+constexpr long long SMALL_GRID_TILES = HF_SMALL_GRID_TILES;
+constexpr long long MID_GRID_TILES = 128;
 
 // Workgroups are dispatched round-robin over the 8 XCDs (workgroup b lands on
 // XCD b % 8), each with its own L2. Renumber tiles so every XCD walks one

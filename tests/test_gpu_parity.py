@@ -245,13 +245,14 @@ def test_dualn_bitwise(n):
     assert np.array_equal(gv, ov) and np.array_equal(gt, ot, equal_nan=True)
 
 
-@pytest.mark.parametrize("kind,nr,nc", [("f64", 520, 600), ("f64", 150, 100)])
-def test_default_and_small_grid_tiles_bitwise(kind, nr, nc):
-    """A grid takes the default stage tiles (f64 32 x 24) when they number
-    at least 160 and 16 x 8 tiles otherwise: 520 x 600 (22 x 19 = 418
-    default tiles) runs the default kernels, 150 x 100 the small ones, each
-    bit for bit with the oracle through the fused driver with friction,
-    rain and point sources and the host loop."""
+@pytest.mark.parametrize("kind,nr,nc", [("f64", 520, 600), ("f64", 260, 300), ("f64", 150, 100)])
+def test_default_mid_and_small_grid_tiles_bitwise(kind, nr, nc):
+    """A grid takes the default f64 stage tiles (32 x 24) when they number
+    at least 160, else 32 x 16 tiles when those number at least 128, else
+    16 x 8: 520 x 600 (418 default tiles), 260 x 300 (110 default, 170 of
+    32 x 16) and 150 x 100 (40 of 32 x 16) run the three kernels, each bit
+    for bit with the oracle through the fused driver with friction, rain
+    and point sources and the host loop."""
     _say(f"large-tile kernels {kind} {nr}x{nc}")
     c = cases.bumpy_wet_dry(nr, nc, 3)
     kw = {}
