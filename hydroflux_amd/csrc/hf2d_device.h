@@ -157,17 +157,21 @@ __device__ inline void block_max_to(double sx, double sy, double (*red)[16], uns
     if (wet_dst) red[2][wave] = wave_wet ? 1.0 : 0.0;
   }
   __syncthreads();
-  if (tid == 0) {
+  if (tid < 64) {  // wave 0 folds the (at most 16) wave partials, one per lane
     const int nw = (blockDim.x + 63) >> 6;
-    bool w = wave_wet;
-    for (int k = 1; k < nw; ++k) {
-      sx = __builtin_fmax(sx, red[0][k]);
-      sy = __builtin_fmax(sy, red[1][k]);
-      if (wet_dst) w = w || red[2][k] != 0.0;
+    const int k = lane < nw ? lane : nw - 1;  // a repeated partial leaves the max unchanged
+    double x = red[0][k], y = red[1][k];
+    const bool w = wet_dst ? __any(red[2][k] != 0.0 ? 1 : 0) != 0 : false;
+#pragma unroll
+    for (int off = 8; off > 0; off >>= 1) {
+      x = __builtin_fmax(x, __shfl_xor(x, off));
+      y = __builtin_fmax(y, __shfl_xor(y, off));
     }
-    atomicMax(dst + 0, hfm::as_u64(sx));
-    atomicMax(dst + 1, hfm::as_u64(sy));
-    if (wet_dst) *wet_dst = w ? 1 : 0;
+    if (tid == 0) {
+      atomicMax(dst + 0, hfm::as_u64(x));
+      atomicMax(dst + 1, hfm::as_u64(y));
+      if (wet_dst) *wet_dst = w ? 1 : 0;
+    }
   }
 }
 
