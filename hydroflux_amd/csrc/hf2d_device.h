@@ -195,7 +195,7 @@ __global__ __launch_bounds__(NT) void stage_kernel(StageArgs<T> a) {
   const int tid = threadIdx.x;
   typename St::Pre pre{};
   if constexpr (St::kPrefetch) St::template prefetch<MODE>(a, gi0, gj0, tid, &pre);
-  St::phase_load(s, a, gi0, gj0, tid, NT);
+  St::template phase_load<NT>(s, a, gi0, gj0, tid);
   __syncthreads();
   St::phase_diffs(s, a, gi0, gj0, tid, NT);
   __syncthreads();
@@ -518,7 +518,7 @@ __global__ __launch_bounds__(NT) void batch_stage_kernel(const StageArgs<T>* __r
     const int64_t gi0 = (int64_t)(it.tile / a.tiles_x) * TH, gj0 = (int64_t)(it.tile % a.tiles_x) * TW;
     typename St::Pre pre{};
     if constexpr (St::kPrefetch) St::template prefetch<MODE>(a, gi0, gj0, tid, &pre);
-    St::phase_load(s, a, gi0, gj0, tid, NT);
+    St::template phase_load<NT>(s, a, gi0, gj0, tid);
     __syncthreads();
     St::phase_diffs(s, a, gi0, gj0, tid, NT);
     __syncthreads();

@@ -535,28 +535,51 @@ template <class T, int TW, int TH> struct Stage {
   // Phase 1: dry flags on the 4-halo; h, z and primitives u, v on the
   // 3-halo (update.rs:405-426, 853-855). Cells outside the domain are
   // "not dry", which clips the +-2 dry windows to the domain as the
-  // reference does.
-  static HF_HD void phase_load(S& s, const StageArgs<T>& a, int64_t gi0, int64_t gj0, int tid, int nt) {
+  // reference does. All global loads of a thread's NT-strided items are
+  // issued before any is used (momentum too, used only where h > H_VEL),
+  // so the phase waits for memory once rather than once per item and
+  // again for the momentum.
+  template <int NT>
+  static HF_HD void phase_load(S& s, const StageArgs<T>& a, int64_t gi0, int64_t gj0, int tid) {
+    constexpr int RN = (S::N4 + NT - 1) / NT;
     const T Z = cst<T>(0.0);
     const int64_t ci = gi0 + TH / 2 < a.nr ? gi0 + TH / 2 : a.nr - 1;
     const int64_t cj = gj0 + TW / 2 < a.nc ? gj0 + TW / 2 : a.nc - 1;
     const double tdat = tile_datum(a, ci, cj);
-    for (int k = tid; k < S::N4; k += nt) {
+    T lh[RN], lz[RN], lhu[RN], lhv[RN];
+#pragma unroll
+    for (int r = 0; r < RN; ++r) {
+      const int k = tid + r * NT;
+      const int li = k / S::W4 - 4, lj = k % S::W4 - 4;
+      const int64_t gi = gi0 + li, gj = gj0 + lj;
+      if (k < S::N4 && gi >= 0 && gi < a.nr && gj >= 0 && gj < a.nc) {
+        const int64_t idx = gi * a.nc + gj;
+        lh[r] = a.ih.ld(idx);
+        if (li >= -3 && li < TH + 3 && lj >= -3 && lj < TW + 3) {
+          lz[r] = load_bed(a, gi, gj, idx, tdat);
+          lhu[r] = a.ihu.ld(idx);
+          lhv[r] = a.ihv.ld(idx);
+        }
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < RN; ++r) {
+      const int k = tid + r * NT;
+      if (k >= S::N4) break;
       const int li = k / S::W4 - 4, lj = k % S::W4 - 4;
       const int64_t gi = gi0 + li, gj = gj0 + lj;
       const bool inside = gi >= 0 && gi < a.nr && gj >= 0 && gj < a.nc;
       const bool in3 = li >= -3 && li < TH + 3 && lj >= -3 && lj < TW + 3;
       if (inside) {
-        const int64_t idx = gi * a.nc + gj;
-        T hh = a.ih.ld(idx);
+        const T hh = lh[r];
         s.dry[k] = val(hh) <= R(H_DRY);
         if (in3) {
           const int k3 = S::i3(li, lj);
           s.h[k3] = hh;
-          s.z[k3] = load_bed(a, gi, gj, idx, tdat);
+          s.z[k3] = lz[r];
           if (val(hh) > R(H_VEL)) {
-            s.u[k3] = div(a.ihu.ld(idx), hh);
-            s.v[k3] = div(a.ihv.ld(idx), hh);
+            s.u[k3] = div(lhu[r], hh);
+            s.v[k3] = div(lhv[r], hh);
           } else {
             s.u[k3] = Z;
             s.v[k3] = Z;

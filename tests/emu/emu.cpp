@@ -103,7 +103,7 @@ template <class T, int TW, int TH>
 void launch(const StageArgs<T>& a, int mode, unsigned long long* dst) {
   using St = Stage<T, TW, TH>;
   std::unique_ptr<typename St::S> s(new typename St::S());
-  const int NT = 256;
+  constexpr int NT = 256;
   const int64_t nb = a.tile_list ? *a.n_work : (int64_t)a.tiles_x * a.tiles_y;
   double gx = 0.0, gy = 0.0;
   double dt;
@@ -118,7 +118,7 @@ void launch(const StageArgs<T>& a, int mode, unsigned long long* dst) {
       else
         St::template prefetch<MODE_FINAL_RK2>(a, gi0, gj0, t, &pre[t]);
     }
-    for (int t = 0; t < NT; ++t) St::phase_load(*s, a, gi0, gj0, t, NT);
+    for (int t = 0; t < NT; ++t) St::template phase_load<NT>(*s, a, gi0, gj0, t);
     for (int t = 0; t < NT; ++t) St::phase_diffs(*s, a, gi0, gj0, t, NT);
     for (int t = 0; t < NT; ++t) St::phase_faces(*s, a, gi0, gj0, t, NT);
     for (int t = 0; t < NT; ++t) St::phase_alpha(*s, a, gi0, gj0, t, NT, dt);
