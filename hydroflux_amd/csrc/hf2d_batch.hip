@@ -35,7 +35,9 @@ struct CfgInfo {
 // The autotuner's candidates per kind: the 256-cell tile shapes
 // (16x16, 32x8, 64x4, 128x2) and the larger high-occupancy tiles.
 const CfgInfo CFG_F32[] = {{32, 32, 1024}, {32, 16, 512}, {16, 16, 256}, {32, 8, 256}, {64, 4, 256}, {128, 2, 256}};
-const CfgInfo CFG_F64[] = {{32, 24, 1024}, {32, 16, 1024}, {16, 16, 256}, {32, 8, 256}};
+// FP64 32 x 24 with 1024 threads is limited to 128 VGPRs and spills in the
+// persistent batch loop; 768 threads (170 VGPRs) does not.
+const CfgInfo CFG_F64[] = {{32, 24, 768}, {32, 24, 1024}, {32, 16, 1024}, {16, 16, 256}, {32, 8, 256}};
 const CfgInfo CFG_DUAL[] = {{16, 8, 512}, {16, 16, 1024}, {16, 8, 256}};
 
 template <class T> struct Cfgs;
@@ -45,7 +47,7 @@ template <> struct Cfgs<float> {
 };
 template <> struct Cfgs<double> {
   static const CfgInfo* list() { return CFG_F64; }
-  static constexpr int n = 4;
+  static constexpr int n = 5;
 };
 template <> struct Cfgs<dual> {
   static const CfgInfo* list() { return CFG_DUAL; }
@@ -96,17 +98,19 @@ template <> int occupancy_cfg<float>(int ci) {
 template <> int launch_cfg<double>(int ci, hipStream_t st, const StageArgs<double>* d1, const StageArgs<double>* d2,
                                    const WorkItem* w, int n, int g) {
   switch (ci) {
-    case 0: return launch_pair<double, 32, 24, 1024>(st, d1, d2, w, n, g);
-    case 1: return launch_pair<double, 32, 16, 1024>(st, d1, d2, w, n, g);
-    case 2: return launch_pair<double, 16, 16, 256>(st, d1, d2, w, n, g);
+    case 0: return launch_pair<double, 32, 24, 768>(st, d1, d2, w, n, g);
+    case 1: return launch_pair<double, 32, 24, 1024>(st, d1, d2, w, n, g);
+    case 2: return launch_pair<double, 32, 16, 1024>(st, d1, d2, w, n, g);
+    case 3: return launch_pair<double, 16, 16, 256>(st, d1, d2, w, n, g);
     default: return launch_pair<double, 32, 8, 256>(st, d1, d2, w, n, g);
   }
 }
 template <> int occupancy_cfg<double>(int ci) {
   switch (ci) {
-    case 0: return occupancy_blocks<double, 32, 24, 1024>();
-    case 1: return occupancy_blocks<double, 32, 16, 1024>();
-    case 2: return occupancy_blocks<double, 16, 16, 256>();
+    case 0: return occupancy_blocks<double, 32, 24, 768>();
+    case 1: return occupancy_blocks<double, 32, 24, 1024>();
+    case 2: return occupancy_blocks<double, 32, 16, 1024>();
+    case 3: return occupancy_blocks<double, 16, 16, 256>();
     default: return occupancy_blocks<double, 32, 8, 256>();
   }
 }

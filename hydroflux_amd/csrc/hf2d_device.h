@@ -26,7 +26,7 @@ template <class T> struct Shape;
 #define HF_F64_TILE_H 24
 #endif
 #ifndef HF_F64_THREADS
-#define HF_F64_THREADS 1024
+#define HF_F64_THREADS 768
 #endif
 #ifndef HF_XCD_REMAP
 #define HF_XCD_REMAP 1
@@ -130,9 +130,6 @@ template <> struct Shape<float> {
   static constexpr int TW = HF_F32_TILE_W, TH = HF_F32_TILE_H, NT = HF_F32_THREADS;
 };
 
-// Workgroup maxima of the CFL wave speeds into dst (64-bit atomic max of
-// the non-negative doubles' bit patterns). With wet_dst (then red has 3
-// rows), also whether any thread saw a wet output cell, one byte.
 // CFL maxima slots of the driver's final stage: 64-bit atomic maxima of one
 // address from every workgroup serialize (device-scope atomics are resolved
 // outside the per-XCD L2s), so the workgroups of a launch spread over
@@ -141,8 +138,10 @@ template <> struct Shape<float> {
 constexpr int WS_SLOTS = 512;
 constexpr int WS_STRIDE = 16;  // u64 per slot
 
-__device__ inline void block_max_to(double sx, double sy, double (*red)[16], unsigned long long* dst,
-                                    bool wet = false, unsigned char* wet_dst = nullptr) {
+// Workgroup maxima of (sx, sy) and whether any thread saw a wet cell,
+// valid in thread 0 (red: per-wave partials, 3 rows when with_wet).
+__device__ inline void block_max(double sx, double sy, double (*red)[16], bool wet, bool with_wet, double* bx,
+                                 double* by, bool* bw) {
   const int tid = threadIdx.x;
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
@@ -150,28 +149,40 @@ __device__ inline void block_max_to(double sx, double sy, double (*red)[16], uns
     sy = __builtin_fmax(sy, __shfl_xor(sy, off));
   }
   const int wave = tid >> 6, lane = tid & 63;
-  const bool wave_wet = wet_dst ? __any(wet ? 1 : 0) != 0 : false;
+  const bool wave_wet = with_wet ? __any(wet ? 1 : 0) != 0 : false;
   if (lane == 0) {
     red[0][wave] = sx;
     red[1][wave] = sy;
-    if (wet_dst) red[2][wave] = wave_wet ? 1.0 : 0.0;
+    if (with_wet) red[2][wave] = wave_wet ? 1.0 : 0.0;
   }
   __syncthreads();
   if (tid < 64) {  // wave 0 folds the (at most 16) wave partials, one per lane
     const int nw = (blockDim.x + 63) >> 6;
     const int k = lane < nw ? lane : nw - 1;  // a repeated partial leaves the max unchanged
     double x = red[0][k], y = red[1][k];
-    const bool w = wet_dst ? __any(red[2][k] != 0.0 ? 1 : 0) != 0 : false;
+    const bool w = with_wet ? __any(red[2][k] != 0.0 ? 1 : 0) != 0 : false;
 #pragma unroll
     for (int off = 8; off > 0; off >>= 1) {
       x = __builtin_fmax(x, __shfl_xor(x, off));
       y = __builtin_fmax(y, __shfl_xor(y, off));
     }
-    if (tid == 0) {
-      atomicMax(dst + 0, hfm::as_u64(x));
-      atomicMax(dst + 1, hfm::as_u64(y));
-      if (wet_dst) *wet_dst = w ? 1 : 0;
-    }
+    *bx = x;
+    *by = y;
+    *bw = w;
+  }
+}
+
+// block_max into dst (64-bit atomic max of the non-negative doubles' bit
+// patterns) and, with wet_dst, the wet flag (one byte).
+__device__ inline void block_max_to(double sx, double sy, double (*red)[16], unsigned long long* dst,
+                                    bool wet = false, unsigned char* wet_dst = nullptr) {
+  double x = 0.0, y = 0.0;
+  bool w = false;
+  block_max(sx, sy, red, wet, wet_dst != nullptr, &x, &y, &w);
+  if (threadIdx.x == 0) {
+    atomicMax(dst + 0, hfm::as_u64(x));
+    atomicMax(dst + 1, hfm::as_u64(y));
+    if (wet_dst) *wet_dst = w ? 1 : 0;
   }
 }
 
@@ -510,8 +521,17 @@ __global__ __launch_bounds__(NT) void batch_stage_kernel(const StageArgs<T>* __r
   using St = hf::Stage<T, TW, TH>;
   __shared__ typename St::S s;
   const int tid = threadIdx.x;
+  // CFL maxima of consecutive items of one scenario are combined here
+  // (thread 0) and reach the scenario's slot in one atomic max per run of
+  // items, not one per tile: the unsigned max of bit patterns, as the
+  // atomics compute it.
+  unsigned long long* pend = nullptr;
+  unsigned long long px = 0, py = 0;
   for (int w = blockIdx.x; w < n_work; w += gridDim.x) {
-    const WorkItem it = work[w];
+    // Work items are in scenario and tile order; the grid is a multiple of
+    // 8 workgroups or one item each, so item w stays on XCD w % 8 and the
+    // permutation gives each XCD a contiguous band, as for single grids.
+    const WorkItem it = work[tile_of_block((unsigned)w, (unsigned)n_work)];
     const StageArgs<T>& a = descs[it.slot];
     double dt;
     if (!hf::stage_prologue(a, it.tile == 0 && tid == 0, &dt)) continue;
@@ -530,10 +550,30 @@ __global__ __launch_bounds__(NT) void batch_stage_kernel(const StageArgs<T>* __r
     bool wet = false;
     St::template phase_update<MODE>(s, a, gi0, gj0, tid, NT, dt, &wsx, &wsy, &wet, St::kPrefetch ? &pre : nullptr);
     if (MODE == hf::MODE_FINAL_RK2 && a.wave_speed) {
-      Ctrl* c = a.ctrl;
-      block_max_to(wsx, wsy, s.red, &c->ws[c->parity ^ 1][0]);
+      double bx = 0.0, by = 0.0;
+      bool bw = false;
+      block_max(wsx, wsy, s.red, false, false, &bx, &by, &bw);
+      if (tid == 0) {
+        Ctrl* c = a.ctrl;
+        unsigned long long* dst = &c->ws[c->parity ^ 1][0];
+        if (dst != pend) {
+          if (pend) {
+            atomicMax(pend + 0, px);
+            atomicMax(pend + 1, py);
+          }
+          pend = dst;
+          px = py = 0;
+        }
+        const unsigned long long ux = hfm::as_u64(bx), uy = hfm::as_u64(by);
+        px = ux > px ? ux : px;
+        py = uy > py ? uy : py;
+      }
     }
     __syncthreads();  // the next work item overwrites the LDS tile
+  }
+  if (tid == 0 && pend) {
+    atomicMax(pend + 0, px);
+    atomicMax(pend + 1, py);
   }
 }
 
