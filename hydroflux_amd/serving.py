@@ -15,7 +15,7 @@ from __future__ import annotations
 
 import time
 from dataclasses import dataclass, field
-from typing import Dict, List, Optional
+from typing import Callable, Dict, List, Optional
 
 import numpy as np
 
@@ -90,7 +90,9 @@ class Server:
         for b in self.batches.values():
             b.close()
 
-    def run(self, requests: List[Request]) -> ServeReport:
+    def run(self, requests: List[Request], on_done: Optional[Callable[[Request], None]] = None) -> ServeReport:
+        """Serve the requests to completion. on_done(request) is called as
+        each one completes (its result fields set, before it leaves the batch)."""
         pending = sorted(requests, key=lambda r: r.arrival)
         waiting: Dict[str, List[Request]] = {k: [] for k in self.batches}
         resident: Dict[str, Dict[int, Request]] = {k: {} for k in self.batches}
@@ -141,6 +143,8 @@ class Server:
                             else:
                                 r.result = out
                         cell_updates += r.spec.n_cells * r.steps
+                        if on_done is not None:
+                            on_done(r)
                         b.remove(slot)
                         del resident[k][slot]
         wall = time.perf_counter() - t0
