@@ -268,6 +268,48 @@ def test_default_mid_and_small_grid_tiles_bitwise(kind, nr, nc):
     assert np.array_equal(g.get_state(), o.get_state())
 
 
+RANDOM_SHAPES = [(23, 17), (150, 100), (260, 300), (520, 600), (61, 389), (300, 41), (97, 211), (400, 200)]
+
+
+@pytest.mark.parametrize("seed", range(len(RANDOM_SHAPES)))
+def test_randomized_configurations_bitwise(seed):
+    """Seeded random configurations across the three f64 tile shapes and
+    the Dual kind: boundary kinds on each side (Discharge q, Depth h),
+    rain, point sources and sinks, friction dry tolerance. The fused driver,
+    then the host loop (boundary-aware CFL, SSP-RK2, friction, rain, point
+    sources), bit for bit with the oracle."""
+    rng = np.random.default_rng(4000 + seed)
+    nr, nc = RANDOM_SHAPES[seed]
+    kind = "dual" if seed % 3 == 2 else "f64"
+    c = cases.bumpy_wet_dry(nr, nc, seed)
+    kinds = tuple(int(k) for k in rng.choice([cases.WALL, cases.TRANS, cases.DISCHARGE, cases.DEPTH], size=4))
+    c.bc_kinds = kinds
+    c.bc_params = tuple(0.4 if k == cases.DISCHARGE else (0.7 if k == cases.DEPTH else 0.0) for k in kinds)
+    rain = float(rng.choice([0.0, 2e-4]))
+    srcs = [(int(rng.integers(nr)), int(rng.integers(nc)), float(rng.uniform(-0.05, 0.3)))
+            for _ in range(int(rng.integers(0, 3)))]
+    tol = float(rng.choice([1e-6, 1e-9]))
+    _say(f"random config {seed}: {kind} {nr}x{nc} bcs {kinds} rain {rain} sources {len(srcs)} tol {tol}")
+    kw = dict(manning_tangent=c.manning) if kind == "dual" else {}
+    g = c.load(_hf().Grid2D(c.nr, c.nc, c.dx, c.dy, kind=kind), **kw)
+    o = c.load(OracleSim(c.nr, c.nc, c.dx, c.dy, dual=kind == "dual"), **kw)
+    for s in (g, o):
+        s.configure(cfl=0.4, max_dt=0.05, rain_rate=rain, sources=srcs, friction_dry_tol=tol)
+        s.run_steps(12)
+        for _ in range(3):
+            dt = s.cfl_time_step(0.4, True)
+            s.ssprk2_step(dt)
+            s.manning_friction_step(dt, tol)
+            s.apply_rain(rain, dt)
+            s.apply_point_sources(srcs, dt)
+    assert g.time() == o.time()
+    if kind == "dual":
+        (gv, gt), (ov, ot) = g.get_state(), o.get_state()
+        assert np.array_equal(gv, ov) and np.array_equal(gt, ot, equal_nan=True)
+    else:
+        assert np.array_equal(g.get_state(), o.get_state())
+
+
 @pytest.mark.parametrize("between", ["synchronize", "enqueue_sync_set_state"])
 def test_synchronize_does_not_validate_stale_cfl_slots(between):
     """hf2d_synchronize records the driver's CFL maxima only after enqueued
