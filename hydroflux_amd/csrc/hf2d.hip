@@ -418,6 +418,7 @@ template <class T> int enqueue_step(hf2d_sim* s) {
   b.ws_part = s->ws_part;
   const bool series = s->series_host[0].n > 0 || s->series_host[1].n > 0;
   b.use_series = series ? 1 : 0;
+  a.series = series ? s->series_dev : nullptr;
   if (list_usable(s)) {
     int rc = enqueue_compact<T>(s);
     if (rc) return rc;
@@ -431,10 +432,6 @@ template <class T> int enqueue_step(hf2d_sim* s) {
   if (s->cfg.integrator == HF2D_SSPRK2) {
     int rc = launch_stage<T>(s, a, hf::MODE_STAGE, 0);
     if (rc) return rc;
-    if (series) {
-      hipLaunchKernelGGL(series_kernel, dim3(1), dim3(1), 0, s->stream, s->ctrl, s->series_dev);
-      HIPCHK(s, hipGetLastError());
-    }
     rc = launch_stage<T>(s, b, hf::MODE_FINAL_RK2, 1);
     if (rc) return rc;
   } else {
@@ -442,10 +439,6 @@ template <class T> int enqueue_step(hf2d_sim* s) {
     // tail copies it back with forcing and the CFL maxima.
     int rc = launch_stage<T>(s, a, hf::MODE_STAGE, 0);
     if (rc) return rc;
-    if (series) {
-      hipLaunchKernelGGL(series_kernel, dim3(1), dim3(1), 0, s->stream, s->ctrl, s->series_dev);
-      HIPCHK(s, hipGetLastError());
-    }
     StageArgs<T> t = b;
     t.ih = fld<T>(s->st[oth][0], s->tg[oth][0]);
     t.ihu = fld<T>(s->st[oth][1], s->tg[oth][1]);
@@ -471,14 +464,18 @@ template <class T> int enqueue_step(hf2d_sim* s) {
       s->ev_used += 2;
     }
   }
-  hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(WS_SLOTS), 0, s->stream, s->ctrl, s->ws_part);
-  HIPCHK(s, hipGetLastError());
-  if (s->n_gauges > 0) {
-    hipLaunchKernelGGL((gauge_kernel<T>), dim3(1), dim3(64), 0, s->stream, s->ctrl,
-                       fld<T>(s->st[s->cur][0], s->tg[s->cur][0]), s->gauge_cells, s->n_gauges, s->gauge_buf,
-                       s->gauge_count, s->gauge_cap, 0);
-    HIPCHK(s, hipGetLastError());
+  GaugeArgs g{};
+  if (s->n_gauges > 0) {  // the value plane of the step's output depth
+    g.h = s->st[s->cur][0];
+    g.f32 = s->kind == HF2D_F32 ? 1 : 0;
+    g.cells = s->gauge_cells;
+    g.n = s->n_gauges;
+    g.buf = s->gauge_buf;
+    g.count = s->gauge_count;
+    g.cap = s->gauge_cap;
   }
+  hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(WS_SLOTS), 0, s->stream, s->ctrl, s->ws_part, g);
+  HIPCHK(s, hipGetLastError());
   return HF2D_OK;
 }
 

@@ -274,19 +274,6 @@ __global__ __launch_bounds__(256) void tile_wet_init_kernel(Fld<T> h, int64_t nr
   }
 }
 
-// The driver's forcing series at the clock after the step about to finish
-// (t + dt, dt published by the first stage), into the control block for
-// the final stage's forcing. One thread.
-__global__ void series_kernel(Ctrl* c, const SeriesDesc* s) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  if (c->status != 0 || c->done != 0) return;
-  const double t_new = hf::clock_after(c, c->dt);
-  for (int k = 0; k < 2; ++k) {
-    c->series_on[k] = s[k].n > 0 ? 1 : 0;
-    if (s[k].n > 0) c->series_now[k] = hf::series_at(s[k], t_new);
-  }
-}
-
 // Gauges: append (t, h at each gauge cell) as row *count of buf (rows of
 // n + 1 doubles) after a step that advanced the clock, or unconditionally
 // with force (the initial row). Rows past cap are dropped.
@@ -365,10 +352,23 @@ template <class T> __global__ void point_source_kernel(StageArgs<T> a, double dt
   }
 }
 
+// Gauge cells of a driver (finalize_kernel): the depth value plane (FP64,
+// or FP32 when f32 is set) and the row buffer of gauge_kernel.
+struct GaugeArgs {
+  const void* h;
+  int f32;
+  const int64_t* cells;
+  int n;
+  double* buf;
+  long long* count;
+  long long cap;
+};
+
 // End of a driver step, one workgroup of WS_SLOTS threads: fold the final
 // stage's CFL slots (part, nullable) into ctrl->ws[parity ^ 1] and clear
-// them, then advance the clock and flip the slot.
-__global__ __launch_bounds__(WS_SLOTS) void finalize_kernel(Ctrl* c, unsigned long long* part) {
+// them, then advance the clock and flip the slot. With g.n > 0 it then
+// appends the gauge row (t, h at each gauge cell) as gauge_kernel does.
+__global__ __launch_bounds__(WS_SLOTS) void finalize_kernel(Ctrl* c, unsigned long long* part, GaugeArgs g) {
   __shared__ unsigned long long red[2][WS_SLOTS / 64];
   if (part) {
     const int t = threadIdx.x;
@@ -400,19 +400,35 @@ __global__ __launch_bounds__(WS_SLOTS) void finalize_kernel(Ctrl* c, unsigned lo
       w[1] = my > w[1] ? my : w[1];
     }
   }
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  if (c->status != 0 || c->done != 0) return;
-  if (c->compensated) {  // FP32 precision form: Kahan-compensated clock
-    const double y = c->dt - c->t_comp;
-    const double t1 = c->t + y;
-    c->t_comp = (t1 - c->t) - y;
-    c->t = t1;
-  } else {
-    c->t = c->t + c->dt;  // sim.rs:225 (plain sum)
+  if (c->status != 0 || c->done != 0) return;  // the same value in every thread
+  if (threadIdx.x == 0) {
+    if (c->compensated) {  // FP32 precision form: Kahan-compensated clock
+      const double y = c->dt - c->t_comp;
+      const double t1 = c->t + y;
+      c->t_comp = (t1 - c->t) - y;
+      c->t = t1;
+    } else {
+      c->t = c->t + c->dt;  // sim.rs:225 (plain sum)
+    }
+    c->steps = c->steps + 1;
+    c->budget = c->budget - 1;
+    c->parity = c->parity ^ 1;
   }
-  c->steps = c->steps + 1;
-  c->budget = c->budget - 1;
-  c->parity = c->parity ^ 1;
+  if (g.n <= 0) return;
+  __syncthreads();  // the new clock
+  const long long row = *g.count;
+  if (row >= g.cap) return;
+  double* r = g.buf + row * (long long)(g.n + 1);
+  for (int k = (int)threadIdx.x; k <= g.n; k += (int)blockDim.x) {
+    if (k == 0)
+      r[0] = c->t;
+    else if (g.f32)
+      r[k] = (double)static_cast<const float*>(g.h)[g.cells[k - 1]];
+    else
+      r[k] = static_cast<const double*>(g.h)[g.cells[k - 1]];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) *g.count = row + 1;
 }
 
 __global__ void zero_slot_kernel(Ctrl* c) {

@@ -209,8 +209,12 @@ struct StageCommon {
   const int* n_work;
   unsigned char* tile_wet;
   // forcing() takes the rain rate and point-source rate from the control
-  // block's series values (series_kernel, at t + dt) when use_series is set.
+  // block's series values (at t + dt) when use_series is set.
   int use_series;
+  // First driver stage: the lead thread, having published dt, evaluates
+  // these two series (rain, point-source discharge) at t + dt into the
+  // control block for the final stage (null: none).
+  const SeriesDesc* series;
   // FP32 precision form: bed = datum[block] (FP64, one per 2^datum_log2
   // square block of cells) + zdev (FP32 deviation); `z` is unused then.
   const double* datum;
@@ -1270,6 +1274,16 @@ HF_HD double dt_from_ctrl(const Ctrl* c, double cfl, double dx, double dy, doubl
   return dt;
 }
 
+// Forcing-series values at the end of the step about to run (t + dt), read
+// by the final stage's forcing.
+HF_HD void publish_series(Ctrl* c, const SeriesDesc* s, double dt) {
+  const double t_new = clock_after(c, dt);
+  for (int k = 0; k < 2; ++k) {
+    c->series_on[k] = s[k].n > 0 ? 1 : 0;
+    if (s[k].n > 0) c->series_now[k] = series_at(s[k], t_new);
+  }
+}
+
 // Common prologue of every stage launch. Returns false when the launch is
 // a no-op (driver finished or failed); block 0 records why.
 template <class T>
@@ -1304,6 +1318,7 @@ HF_HD bool stage_prologue(const StageArgs<T>& a, bool lead, double* dt_out) {
     c->dt = dt;
     c->ws[c->parity ^ 1][0] = 0ULL;
     c->ws[c->parity ^ 1][1] = 0ULL;
+    if (a.series) publish_series(c, a.series, dt);
   }
   *dt_out = dt;
   return true;

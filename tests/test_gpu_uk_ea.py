@@ -78,6 +78,47 @@ def test_series_and_gauges_bitwise_with_oracle(kind):
         assert np.array_equal(a, r)
 
 
+@pytest.mark.parametrize("kind", ["f64", "f32", "dual", "dual4"])
+def test_gauge_rows_equal_the_state_after_each_step(kind):
+    """Every scalar kind: each recorded row is (clock, depth value at the
+    gauge cells) of the state after that step, one row per step, stepped one
+    at a time and through the captured multi-step graphs; rows past the
+    capacity are dropped."""
+    _say(f"gauge rows vs state {kind}")
+    from hydroflux_amd import Grid2D
+
+    c = cases.mostly_dry(40, 56, 7, west=cases.WALL)
+    cells = [(20, 1), (20, 9), (33, 40), (0, 0)]
+    g = c.load(Grid2D(c.nr, c.nc, c.dx, c.dy, kind=kind))
+    g.configure(cfl=0.4, max_dt=0.5, sources=[(20, 0, 0.0), (33, 40, 0.01)])
+    g.set_forcing_series("sources", (0.0, 0.5, 2.0, 40.0), (0.0, 0.0, 1.0, 1.0), 1.0, 0.3, 1.0)
+    g.set_forcing_series("rain", (0.0, 1.0, 2.0), (0.0, 200.0, 200.0), 1.0, 1e-3, 3600.0)
+    g.set_gauges(cells, 64)
+
+    def depth_now():
+        s = g.get_state()
+        h = (s[0] if isinstance(s, tuple) else s)[..., 0]
+        return np.array([h[r, cc] for r, cc in cells], np.float64)
+
+    expect_t, expect_h = [g.time()[0]], [depth_now()]
+    for _ in range(5):
+        g.step()
+        expect_t.append(g.time()[0])
+        expect_h.append(depth_now())
+    g.run_steps(40)  # graph replays
+    t, steps = g.time()
+    assert steps == 45
+    gt, gh = g.read_gauges()
+    assert gt.shape == (46,) and gh.shape == (46, len(cells))
+    assert np.array_equal(gt[:6], expect_t) and np.array_equal(gh[:6], np.array(expect_h))
+    assert gt[-1] == t and np.array_equal(gh[-1], depth_now())
+    assert np.all(np.diff(gt) > 0) and np.any(gh[-1] > 0.0)
+    g.run_steps(30)
+    gt, gh = g.read_gauges()
+    assert gt.shape == (64,) and g.time()[1] == 75
+    g.close()
+
+
 def _oracle_test4(t_end):
     from hydroflux_amd import uk_ea
     from hydroflux_amd.ascii_grid import read_ascii_grid
