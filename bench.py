@@ -285,11 +285,12 @@ def uk_ea(args):
         cells = g.n_rows * g.n_cols
         g.synchronize()
         t0 = time.perf_counter()
-        g.run_until(U.SIM_TIME)
+        t_end = args.ukea_time if args.ukea_time > 0 else U.SIM_TIME
+        g.run_until(t_end)
         g.synchronize()
         wall = time.perf_counter() - t0
         steps = g.time()[1]
-        out[name] = {"grid": [g.n_rows, g.n_cols], "steps": steps, "wall_s": wall,
+        out[name] = {"grid": [g.n_rows, g.n_cols], "simulated_s": t_end, "steps": steps, "wall_s": wall,
                      "cell_updates_per_s": cells * steps / wall, "us_per_step": wall / steps * 1e6,
                      "reference_cpu_wall_s": ref_s}
         g.close()
@@ -513,6 +514,8 @@ def main():
     ap.add_argument("--sections", default="precision,c1,dry,ukea,huasco,forward,batch,serving",
                     help="comma list of extra sections: precision, c1, dry, ukea, huasco, forward, batch, serving")
     ap.add_argument("--dry-warmup", type=int, default=500)
+    ap.add_argument("--ukea-time", type=float, default=0.0,
+                    help="simulated seconds of the UK EA runs (0: the full 18000 s)")
     ap.add_argument("--acc-steps", type=int, default=2000, help="FP64 steps for the FP32 accuracy check (0: skip)")
     ap.add_argument("--serve-requests", type=int, default=64)
     ap.add_argument("--serve-duration", type=float, nargs=2, default=(30.0, 120.0))

@@ -139,16 +139,46 @@ template <> struct Shape<float> {
 constexpr int WS_SLOTS = 512;
 constexpr int WS_STRIDE = 16;  // u64 per slot
 
+// max(v, v of the lane that DPP control CTRL names), in the lanes of the rows
+// ROW_MASK enables (the others keep v). fmax ignores a NaN operand.
+template <int CTRL, int ROW_MASK = 0xf> __device__ inline double dpp_fmax(double v) {
+  const unsigned long long u = hfm::as_u64(v);
+  const int lo = (int)(unsigned)u, hi = (int)(unsigned)(u >> 32);
+  const int lo2 = __builtin_amdgcn_update_dpp(lo, lo, CTRL, ROW_MASK, 0xf, false);
+  const int hi2 = __builtin_amdgcn_update_dpp(hi, hi, CTRL, ROW_MASK, 0xf, false);
+  return __builtin_fmax(v, hfm::as_f64(((unsigned long long)(unsigned)hi2 << 32) | (unsigned)lo2));
+}
+
+// Max over each row of 16 lanes, in every lane of the row: xor 1, xor 2
+// (quad permutations), then the half-row and row mirrors.
+__device__ inline double row_fmax(double v) {
+  v = dpp_fmax<0xB1>(v);   // quad_perm [1,0,3,2]
+  v = dpp_fmax<0x4E>(v);   // quad_perm [2,3,0,1]
+  v = dpp_fmax<0x141>(v);  // row_half_mirror
+  return dpp_fmax<0x140>(v);  // row_mirror
+}
+
+// Max over the wavefront (all 64 lanes active), uniform: row maxima, then
+// row 1 and 3 take lane 15 / 47 (row_bcast:15), rows 2 and 3 take lane 31
+// (row_bcast:31); lane 63 then holds the maximum.
+__device__ inline double wave_fmax(double v) {
+  v = row_fmax(v);
+  v = dpp_fmax<0x142, 0xA>(v);
+  v = dpp_fmax<0x143, 0xC>(v);
+  const unsigned long long u = hfm::as_u64(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, 63);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), 63);
+  return hfm::as_f64(((unsigned long long)hi << 32) | lo);
+}
+
 // Workgroup maxima of (sx, sy) and whether any thread saw a wet cell,
-// valid in thread 0 (red: per-wave partials, 3 rows when with_wet).
+// valid in thread 0 (red: per-wave partials, 3 rows when with_wet). The
+// lane folds are DPP moves, not LDS permutes.
 __device__ inline void block_max(double sx, double sy, double (*red)[16], bool wet, bool with_wet, double* bx,
                                  double* by, bool* bw) {
   const int tid = threadIdx.x;
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    sx = __builtin_fmax(sx, __shfl_xor(sx, off));
-    sy = __builtin_fmax(sy, __shfl_xor(sy, off));
-  }
+  sx = wave_fmax(sx);
+  sy = wave_fmax(sy);
   const int wave = tid >> 6, lane = tid & 63;
   const bool wave_wet = with_wet ? __any(wet ? 1 : 0) != 0 : false;
   if (lane == 0) {
@@ -157,16 +187,13 @@ __device__ inline void block_max(double sx, double sy, double (*red)[16], bool w
     if (with_wet) red[2][wave] = wave_wet ? 1.0 : 0.0;
   }
   __syncthreads();
-  if (tid < 64) {  // wave 0 folds the (at most 16) wave partials, one per lane
+  if (tid < 64) {  // wave 0 folds the (at most 16) wave partials, one per lane of row 0
     const int nw = (blockDim.x + 63) >> 6;
     const int k = lane < nw ? lane : nw - 1;  // a repeated partial leaves the max unchanged
     double x = red[0][k], y = red[1][k];
     const bool w = with_wet ? __any(red[2][k] != 0.0 ? 1 : 0) != 0 : false;
-#pragma unroll
-    for (int off = 8; off > 0; off >>= 1) {
-      x = __builtin_fmax(x, __shfl_xor(x, off));
-      y = __builtin_fmax(y, __shfl_xor(y, off));
-    }
+    x = row_fmax(x);
+    y = row_fmax(y);
     *bx = x;
     *by = y;
     *bw = w;
