@@ -391,71 +391,115 @@ struct GaugeArgs {
   long long cap;
 };
 
+// max(v, v of the lane that DPP control CTRL names) as unsigned 64-bit
+// integers, in the lanes of the rows ROW_MASK enables (others keep v).
+template <int CTRL, int ROW_MASK = 0xf> __device__ inline unsigned long long dpp_umax(unsigned long long v) {
+  const int lo = (int)(unsigned)v, hi = (int)(unsigned)(v >> 32);
+  const int lo2 = __builtin_amdgcn_update_dpp(lo, lo, CTRL, ROW_MASK, 0xf, false);
+  const int hi2 = __builtin_amdgcn_update_dpp(hi, hi, CTRL, ROW_MASK, 0xf, false);
+  const unsigned long long o = ((unsigned long long)(unsigned)hi2 << 32) | (unsigned)lo2;
+  return o > v ? o : v;
+}
+
+// Unsigned 64-bit maximum over the wavefront (all lanes active), uniform;
+// the same DPP sequence as wave_fmax.
+__device__ inline unsigned long long wave_umax(unsigned long long v) {
+  v = dpp_umax<0xB1>(v);
+  v = dpp_umax<0x4E>(v);
+  v = dpp_umax<0x141>(v);
+  v = dpp_umax<0x140>(v);
+  v = dpp_umax<0x142, 0xA>(v);
+  v = dpp_umax<0x143, 0xC>(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)v, 63);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(v >> 32), 63);
+  return ((unsigned long long)hi << 32) | lo;
+}
+
 // End of a driver step, one workgroup of WS_SLOTS threads: fold the final
 // stage's CFL slots (part, nullable) into ctrl->ws[parity ^ 1] and clear
 // them, then advance the clock and flip the slot. With g.n > 0 it then
 // appends the gauge row (t, h at each gauge cell) as gauge_kernel does.
+// Every load that depends on no other is issued first (the slots, the
+// control block, the gauge row index and depths), so the launch waits for
+// one memory round trip before its stores, not one per step of the chain.
 __global__ __launch_bounds__(WS_SLOTS) void finalize_kernel(Ctrl* c, unsigned long long* part, GaugeArgs g) {
   __shared__ unsigned long long red[2][WS_SLOTS / 64];
+  const int t = threadIdx.x;
+  unsigned long long mx = 0, my = 0;
   if (part) {
-    const int t = threadIdx.x;
-    unsigned long long mx = 0, my = 0;
-    if (t < WS_SLOTS) {
-      mx = part[(size_t)t * WS_STRIDE];
-      my = part[(size_t)t * WS_STRIDE + 1];
-      part[(size_t)t * WS_STRIDE] = 0ULL;
-      part[(size_t)t * WS_STRIDE + 1] = 0ULL;
-    }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-      const unsigned long long ox = __shfl_xor(mx, off), oy = __shfl_xor(my, off);
-      mx = ox > mx ? ox : mx;
-      my = oy > my ? oy : my;
-    }
+    mx = part[(size_t)t * WS_STRIDE];
+    my = part[(size_t)t * WS_STRIDE + 1];
+  }
+  const int status = c->status, done = c->done;  // the same in every thread
+  int parity = 0, compensated = 0;
+  unsigned long long ws[2][2] = {};
+  double ct = 0.0, cdt = 0.0, ccomp = 0.0;
+  long long steps = 0, budget = 0;
+  if (t == 0) {
+    parity = c->parity;
+    compensated = c->compensated;
+    ws[0][0] = c->ws[0][0];
+    ws[0][1] = c->ws[0][1];
+    ws[1][0] = c->ws[1][0];
+    ws[1][1] = c->ws[1][1];
+    ct = c->t;
+    cdt = c->dt;
+    ccomp = c->t_comp;
+    steps = c->steps;
+    budget = c->budget;
+  }
+  long long row = 0;
+  double gv = 0.0;
+  if (g.n > 0) {
+    row = *g.count;
+    if (t >= 1 && t <= g.n)
+      gv = g.f32 ? (double)static_cast<const float*>(g.h)[g.cells[t - 1]]
+                 : static_cast<const double*>(g.h)[g.cells[t - 1]];
+  }
+  if (part) {
+    part[(size_t)t * WS_STRIDE] = 0ULL;
+    part[(size_t)t * WS_STRIDE + 1] = 0ULL;
+    mx = wave_umax(mx);
+    my = wave_umax(my);
     if ((t & 63) == 0) {
       red[0][t >> 6] = mx;
       red[1][t >> 6] = my;
     }
     __syncthreads();
     if (t == 0) {
-      for (int k = 1; k < (int)(blockDim.x >> 6); ++k) {
+      for (int k = 1; k < WS_SLOTS / 64; ++k) {
         mx = red[0][k] > mx ? red[0][k] : mx;
         my = red[1][k] > my ? red[1][k] : my;
       }
-      unsigned long long* w = &c->ws[c->parity ^ 1][0];
-      w[0] = mx > w[0] ? mx : w[0];
-      w[1] = my > w[1] ? my : w[1];
+      const unsigned long long* w = ws[parity ^ 1];
+      c->ws[parity ^ 1][0] = mx > w[0] ? mx : w[0];
+      c->ws[parity ^ 1][1] = my > w[1] ? my : w[1];
     }
   }
-  if (c->status != 0 || c->done != 0) return;  // the same value in every thread
-  if (threadIdx.x == 0) {
-    if (c->compensated) {  // FP32 precision form: Kahan-compensated clock
-      const double y = c->dt - c->t_comp;
-      const double t1 = c->t + y;
-      c->t_comp = (t1 - c->t) - y;
-      c->t = t1;
+  if (status != 0 || done != 0) return;
+  if (t == 0) {
+    if (compensated) {  // FP32 precision form: Kahan-compensated clock
+      const double y = cdt - ccomp;
+      const double t1 = ct + y;
+      c->t_comp = (t1 - ct) - y;
+      ct = t1;
     } else {
-      c->t = c->t + c->dt;  // sim.rs:225 (plain sum)
+      ct = ct + cdt;  // sim.rs:225 (plain sum)
     }
-    c->steps = c->steps + 1;
-    c->budget = c->budget - 1;
-    c->parity = c->parity ^ 1;
+    c->t = ct;
+    c->steps = steps + 1;
+    c->budget = budget - 1;
+    c->parity = parity ^ 1;
   }
-  if (g.n <= 0) return;
-  __syncthreads();  // the new clock
-  const long long row = *g.count;
-  if (row >= g.cap) return;
+  if (g.n <= 0 || row >= g.cap) return;
   double* r = g.buf + row * (long long)(g.n + 1);
-  for (int k = (int)threadIdx.x; k <= g.n; k += (int)blockDim.x) {
-    if (k == 0)
-      r[0] = c->t;
-    else if (g.f32)
-      r[k] = (double)static_cast<const float*>(g.h)[g.cells[k - 1]];
-    else
-      r[k] = static_cast<const double*>(g.h)[g.cells[k - 1]];
-  }
+  if (t == 0) r[0] = ct;
+  if (t >= 1 && t <= g.n) r[t] = gv;
+  for (int k = t + WS_SLOTS; k <= g.n; k += WS_SLOTS)
+    r[k] = g.f32 ? (double)static_cast<const float*>(g.h)[g.cells[k - 1]]
+                 : static_cast<const double*>(g.h)[g.cells[k - 1]];
   __syncthreads();
-  if (threadIdx.x == 0) *g.count = row + 1;
+  if (t == 0) *g.count = row + 1;
 }
 
 __global__ void zero_slot_kernel(Ctrl* c) {
