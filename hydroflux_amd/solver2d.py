@@ -325,9 +325,14 @@ class Grid2D:
     def configure(self, cfl=0.4, integrator=1, cfl_with_bcs=True, max_dt=60.0, friction=True,
                   friction_dry_tol=H_DRY, max_steps=10_000_000, rain_rate=0.0, sources=(), active_tiles=True):
         """SimulationConfig + forcing. active_tiles=False turns off the
-        compacted active-tile list (same results; for timing comparisons)."""
+        compacted active-tile list (same results; for timing comparisons);
+        "always" uses it on grids of any size (by default only above 512
+        tiles)."""
+        if active_tiles not in (True, False, "always"):
+            raise ValueError("active_tiles must be True, False or 'always'")
+        opts = 2 if active_tiles == "always" else 0 if active_tiles else 1
         cfg = HfConfig(float(cfl), int(integrator), 1 if cfl_with_bcs else 0, float(max_dt),
-                       1 if friction else 0, 0 if active_tiles else 1, float(friction_dry_tol), int(max_steps),
+                       1 if friction else 0, opts, float(friction_dry_tol), int(max_steps),
                        float(rain_rate))
         arr, k = _sources(sources)
         rc = lib().hf2d_configure(self._h, C.byref(cfg), arr, k)

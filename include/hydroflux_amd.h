@@ -82,9 +82,12 @@ enum hf2d_integrator { HF2D_FORWARD_EULER = 0, HF2D_SSPRK2 = 1 };
  * been dry, with a dry neighbourhood, for two steps are left out; their
  * state is a fixed point of the step, so the results are the same bits.
  * Used for f64 and f32 grids without rain and with friction off on dry
- * cells (friction_dry_tol >= 1e-6); otherwise every tile runs. This flag
- * turns the list off (for timing comparisons). */
-enum hf2d_option { HF2D_OPT_NO_ACTIVE_TILES = 1 };
+ * cells (friction_dry_tol >= 1e-6) of more than 512 tiles; otherwise every
+ * tile runs. On fewer tiles every tile runs at once, so leaving some out
+ * saves no time and the list's upkeep costs some. HF2D_OPT_NO_ACTIVE_TILES
+ * turns the list off (for timing comparisons); HF2D_OPT_ACTIVE_TILES_ANY_SIZE
+ * uses it whatever the grid's size (for tests of the list on small grids). */
+enum hf2d_option { HF2D_OPT_NO_ACTIVE_TILES = 1, HF2D_OPT_ACTIVE_TILES_ANY_SIZE = 2 };
 
 /* PointSource (source.rs:36-44). */
 typedef struct hf2d_point_source {

@@ -157,7 +157,7 @@ def test_active_tile_list_bitwise(kind, west):
         kw = dict(manning_tangent=c.manning, state_tangent=0.1 * rng.standard_normal((c.nr, c.nc, 3)))
     sims = [c.load(hf.Grid2D(c.nr, c.nc, c.dx, c.dy, kind=kind), **kw) for _ in range(2)]
     src = [(c.nr - 9, 11, 0.05), (20, c.nc - 30, 0.02)]
-    for s, on in zip(sims, (True, False)):
+    for s, on in zip(sims, ("always", False)):
         s.configure(cfl=0.4, max_dt=0.2, rain_rate=0.0, sources=src, active_tiles=on)
         s.run_steps(60)
     assert sims[0].time() == sims[1].time()
@@ -187,8 +187,8 @@ def test_graph_replays_interleaved_with_host_calls_bitwise(rain):
     c = cases.mostly_dry(90, 130, 6, west=cases.DEPTH)
     g, o = _pair(c)
     src = [(c.nr - 9, 11, 0.05)]
-    for s in (g, o):
-        s.configure(cfl=0.4, max_dt=0.2, rain_rate=rain, sources=src)
+    g.configure(cfl=0.4, max_dt=0.2, rain_rate=rain, sources=src, active_tiles="always")
+    o.configure(cfl=0.4, max_dt=0.2, rain_rate=rain, sources=src)
     for s in (g, o):
         s.run_steps(40)           # 2 replays + 8 eager steps
     assert np.array_equal(g.get_state(), o.get_state())
@@ -198,9 +198,11 @@ def test_graph_replays_interleaved_with_host_calls_bitwise(rain):
     assert g.time() == o.time() and np.array_equal(g.get_state(), o.get_state())
     st = g.get_state()
     st[45, 60] = (0.4, 0.1, 0.0)
+    g.set_state(st)
+    g.configure(cfl=0.35, max_dt=0.1, rain_rate=rain, sources=src + [(50, 70, 0.01)], active_tiles="always")
+    o.set_state(st)
+    o.configure(cfl=0.35, max_dt=0.1, rain_rate=rain, sources=src + [(50, 70, 0.01)])
     for s in (g, o):
-        s.set_state(st)
-        s.configure(cfl=0.35, max_dt=0.1, rain_rate=rain, sources=src + [(50, 70, 0.01)])
         s.run_until(s.time()[0] + 3.0)
     assert g.time() == o.time() and np.array_equal(g.get_state(), o.get_state())
     g.set_kernel_timing(True)     # eager path with events, then graphs again
