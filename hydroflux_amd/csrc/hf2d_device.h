@@ -37,9 +37,10 @@ template <class T> struct Shape;
 // the default shape when it gives at least SMALL_GRID_TILES tiles (FP64 on
 // the flood workload: the default wins from 192 tiles on, 16 x 8 at 88);
 // else ShapeMid when that gives at least MID_GRID_TILES tiles; else
-// ShapeSmall. Measured FP64 us/step over full runs, 16x8x512 / 32x16x1024 /
-// 32x24x1024: UK EA Test 4 (400 x 200) 48.8 / 49.6 / 58.9, Test 8A
-// (199 x 481) 58.7 / 48.2 / 59.2, Huasco reach (200 x 67) 25.8 / 36.9 / 45.4.
+// ShapeSmall. Measured FP64 us/step over full runs, 16x8x512 / 32x16x1024:
+// UK EA Test 4 (400 x 200) 40.3 / 36.9, Test 8A (199 x 481) 45.7 / 40.5,
+// Huasco reach (200 x 67) 23.7 / 33.0 (32x24x1024, earlier: 20-40 % slower
+// than the better of the two on all three).
 template <class T> struct ShapeSmall {
   static constexpr bool on = false;
 };
@@ -61,8 +62,11 @@ template <> struct ShapeSmall<float> {
 #ifndef HF_SMALL_GRID_TILES
 #define HF_SMALL_GRID_TILES 160
 #endif
+#ifndef HF_MID_GRID_TILES
+#define HF_MID_GRID_TILES 128
+#endif
 constexpr long long SMALL_GRID_TILES = HF_SMALL_GRID_TILES;
-constexpr long long MID_GRID_TILES = 128;
+constexpr long long MID_GRID_TILES = HF_MID_GRID_TILES;
 
 // Workgroups are dispatched round-robin over the 8 XCDs (workgroup b lands on
 // XCD b % 8), each with its own L2. Renumber tiles so every XCD walks one
