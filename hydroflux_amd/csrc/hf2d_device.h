@@ -306,8 +306,9 @@ __global__ __launch_bounds__(256) void tile_wet_init_kernel(Fld<T> h, int64_t nr
 }
 
 // Gauges: append (t, h at each gauge cell) as row *count of buf (rows of
-// n + 1 doubles) after a step that advanced the clock, or unconditionally
-// with force (the initial row). Rows past cap are dropped.
+// n + 1 doubles), unconditionally with force, else only when the driver is
+// live. The driver uses it for the initial row; finalize_kernel appends the
+// row of every step. Rows past cap are dropped.
 template <class T>
 __global__ __launch_bounds__(64) void gauge_kernel(const Ctrl* c, Fld<T> h, const int64_t* cells, int n,
                                                    double* buf, long long* count, long long cap, int force) {

@@ -119,6 +119,30 @@ def test_gauge_rows_equal_the_state_after_each_step(kind):
     g.close()
 
 
+@pytest.mark.parametrize("kind", ["f64", "f32"])
+def test_gauge_rows_with_more_gauges_than_threads(kind):
+    """700 gauges: the row is written by more columns than the recording
+    workgroup has threads (512), so some threads write several."""
+    _say(f"700 gauges {kind}")
+    from hydroflux_amd import Grid2D
+
+    c = cases.mostly_dry(40, 56, 9, west=cases.WALL)
+    rng = np.random.default_rng(3)
+    cells = [(int(r), int(cc)) for r, cc in zip(rng.integers(0, c.nr, 700), rng.integers(0, c.nc, 700))]
+    g = c.load(Grid2D(c.nr, c.nc, c.dx, c.dy, kind=kind))
+    g.configure(cfl=0.4, max_dt=0.5, rain_rate=2e-5, sources=[(20, 0, 0.2)])
+    g.set_gauges(cells, 100)
+    g.run_steps(3)   # eager
+    h3 = g.get_state()[..., 0]
+    g.run_steps(32)  # graph replays
+    h = g.get_state()[..., 0]
+    gt, gh = g.read_gauges()
+    assert gh.shape == (36, 700) and gt[-1] == g.time()[0]
+    assert np.array_equal(gh[3], [h3[r, cc] for r, cc in cells])
+    assert np.array_equal(gh[-1], [h[r, cc] for r, cc in cells])
+    g.close()
+
+
 def _oracle_test4(t_end):
     from hydroflux_amd import uk_ea
     from hydroflux_amd.ascii_grid import read_ascii_grid
