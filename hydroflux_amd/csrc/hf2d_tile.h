@@ -647,12 +647,14 @@ template <class T, int TW, int TH> struct Stage {
       } else {
         const int kk = k - S::NDX - S::NDY;
         const int li = kk / S::WND - 2, lj = kk % S::WND - 2;
-        bool nx = false, ny = false;
+        // bitwise or, not ||: the ten flag reads issue together instead of
+        // one branch (and LDS round trip) per read
+        unsigned nx = 0, ny = 0;
         for (int d = -2; d <= 2; ++d) {
-          nx = nx || (s.dry[S::i4(li, lj + d)] != 0);
-          ny = ny || (s.dry[S::i4(li + d, lj)] != 0);
+          nx |= s.dry[S::i4(li, lj + d)];
+          ny |= s.dry[S::i4(li + d, lj)];
         }
-        s.nd[S::i3(li, lj)] = (unsigned char)((nx ? 1 : 0) | (ny ? 2 : 0));
+        s.nd[S::i3(li, lj)] = (unsigned char)((nx != 0 ? 1 : 0) | (ny != 0 ? 2 : 0));
       }
     }
   }
@@ -1096,9 +1098,11 @@ template <class T, int TW, int TH> struct Stage {
       const int c3 = S::i3(li, lj);
       const T h_old = s.h[c3];
       T nh, nhu, nhv;
-      if (s.dry[S::i4(li, lj)] && gi > 0 && gi + 1 < a.nr && gj > 0 && gj + 1 < a.nc &&
-          s.dry[S::i4(li - 1, lj)] && s.dry[S::i4(li + 1, lj)] && s.dry[S::i4(li, lj - 1)] &&
-          s.dry[S::i4(li, lj + 1)]) {
+      // the five flags (all inside the 4-cell halo) read together, not
+      // behind one short-circuit branch each
+      const unsigned dry5 = s.dry[S::i4(li, lj)] & s.dry[S::i4(li - 1, lj)] & s.dry[S::i4(li + 1, lj)] &
+                            s.dry[S::i4(li, lj - 1)] & s.dry[S::i4(li, lj + 1)];
+      if (dry5 != 0 && gi > 0 && gi + 1 < a.nr && gj > 0 && gj + 1 < a.nc) {
         nh = h_old;
         nhu = Z;
         nhv = Z;
