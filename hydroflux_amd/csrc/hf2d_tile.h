@@ -760,7 +760,7 @@ template <class T, int TW, int TH> struct Stage {
           well_balanced_face(gh, ghu, ghv, h, hu, hv, zf, &m, &na, &nb, &tg);
         else
           well_balanced_face(h, hu, hv, gh, ghu, ghv, zf, &m, &na, &nb, &tg);
-      } else if (!(s.dry[S::i4(li, fj - 1)] && s.dry[S::i4(li, fj)])) {
+      } else if ((s.dry[S::i4(li, fj - 1)] & s.dry[S::i4(li, fj)]) == 0) {  // not both dry
         const R half_dx = R(0.5 * a.dx);
         const int l = S::i3(li, fj - 1), r = S::i3(li, fj);
         T sle, slu, slv, sre, sru, srv;
@@ -818,7 +818,7 @@ template <class T, int TW, int TH> struct Stage {
           well_balanced_face(gh, ghv, ghu, h, hv, hu, zf, &m, &na, &nb, &tg);
         else
           well_balanced_face(h, hv, hu, gh, ghv, ghu, zf, &m, &na, &nb, &tg);
-      } else if (!(s.dry[S::i4(fi - 1, lj)] && s.dry[S::i4(fi, lj)])) {
+      } else if ((s.dry[S::i4(fi - 1, lj)] & s.dry[S::i4(fi, lj)]) == 0) {  // not both dry
         const R half_dy = R(0.5 * a.dy);
         const int t = S::i3(fi - 1, lj), b = S::i3(fi, lj);
         T ste, stu, stv, sbe, sbu, sbv;
@@ -905,7 +905,7 @@ template <class T, int TW, int TH> struct Stage {
     *nb = s.fxb[k];
     *tg = s.fxt[k];
     const int64_t gjf = gj0 + fj;
-    if (gjf > 0 && gjf < a.nc && s.dry[S::i4(li, fj - 1)] && s.dry[S::i4(li, fj)]) return;
+    if ((s.dry[S::i4(li, fj - 1)] & s.dry[S::i4(li, fj)]) != 0 && gjf > 0 && gjf < a.nc) return;
     T al;
     if (gjf == 0)
       al = val(*m) < R(0) ? s.alpha[S::ia(li, fj)] : cst<T>(1.0);
@@ -929,7 +929,7 @@ template <class T, int TW, int TH> struct Stage {
     *nb = s.fyb[k];
     *tg = s.fyt[k];
     const int64_t gif = gi0 + fi;
-    if (gif > 0 && gif < a.nr && s.dry[S::i4(fi - 1, lj)] && s.dry[S::i4(fi, lj)]) return;
+    if ((s.dry[S::i4(fi - 1, lj)] & s.dry[S::i4(fi, lj)]) != 0 && gif > 0 && gif < a.nr) return;
     T al;
     if (gif == 0)
       al = val(*m) < R(0) ? s.alpha[S::ia(fi, lj)] : cst<T>(1.0);
