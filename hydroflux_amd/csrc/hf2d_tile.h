@@ -941,16 +941,16 @@ template <class T, int TW, int TH> struct Stage {
     *nb = s.fxb[k];
     *tg = s.fxt[k];
     const int64_t gjf = gj0 + fj;
+    // both upstream candidates read with the face, not after its sign is known
+    const T al_l = s.alpha[S::ia(li, fj - 1)], al_r = s.alpha[S::ia(li, fj)];
     if ((s.dry[S::i4(li, fj - 1)] & s.dry[S::i4(li, fj)]) != 0 && gjf > 0 && gjf < a.nc) return;
     T al;
     if (gjf == 0)
-      al = val(*m) < R(0) ? s.alpha[S::ia(li, fj)] : cst<T>(1.0);
+      al = val(*m) < R(0) ? al_r : cst<T>(1.0);
     else if (gjf == a.nc)
-      al = val(*m) > R(0) ? s.alpha[S::ia(li, fj - 1)] : cst<T>(1.0);
-    else if (val(*m) > R(0))
-      al = s.alpha[S::ia(li, fj - 1)];
+      al = val(*m) > R(0) ? al_l : cst<T>(1.0);
     else
-      al = s.alpha[S::ia(li, fj)];
+      al = val(*m) > R(0) ? al_l : al_r;
     *m = *m * al;
     *na = *na * al;
     *nb = *nb * al;
@@ -965,16 +965,15 @@ template <class T, int TW, int TH> struct Stage {
     *nb = s.fyb[k];
     *tg = s.fyt[k];
     const int64_t gif = gi0 + fi;
+    const T al_t = s.alpha[S::ia(fi - 1, lj)], al_b = s.alpha[S::ia(fi, lj)];
     if ((s.dry[S::i4(fi - 1, lj)] & s.dry[S::i4(fi, lj)]) != 0 && gif > 0 && gif < a.nr) return;
     T al;
     if (gif == 0)
-      al = val(*m) < R(0) ? s.alpha[S::ia(fi, lj)] : cst<T>(1.0);
+      al = val(*m) < R(0) ? al_b : cst<T>(1.0);
     else if (gif == a.nr)
-      al = val(*m) > R(0) ? s.alpha[S::ia(fi - 1, lj)] : cst<T>(1.0);
-    else if (val(*m) > R(0))
-      al = s.alpha[S::ia(fi - 1, lj)];
+      al = val(*m) > R(0) ? al_t : cst<T>(1.0);
     else
-      al = s.alpha[S::ia(fi, lj)];
+      al = val(*m) > R(0) ? al_t : al_b;
     *m = *m * al;
     *na = *na * al;
     *nb = *nb * al;
