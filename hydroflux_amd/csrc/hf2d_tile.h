@@ -662,12 +662,13 @@ template <class T, int TW, int TH> struct Stage {
   // The slope of a cell from the two face differences l, r that bracket it:
   // zero, one-sided on the first / last cell, else minmod(l, r).
   //
-  // FP32 reads every candidate before the choice and selects, so the
-  // difference loads do not wait behind a branch on the near-dry flag
-  // (+3 % on the FP32 step). FP64 and the dual kinds keep the branches:
-  // there the extra live values cost more (FP64 -0.5 %; Dual's final stage
-  // spilled, +65 % time).
+  // The select form reads every candidate before the choice, so the
+  // difference loads do not wait behind a branch on the near-dry flag.
+  // Measured on MI355X: FP32 takes it everywhere (+3.4 %); FP64 only for
+  // the update phase's bed-slope slopes (+0.5 %; in the face phase -1 %);
+  // the dual kinds nowhere (Dual's final stage then spills, +65 % time).
   static constexpr bool kSelectSlopes = std::is_same<T, float>::value;
+  static constexpr bool kSelectEta = std::is_same<T, float>::value || std::is_same<T, double>::value;
   static HF_HD T pick_slope(bool zero, bool first, bool last, T l, T r) {
     const T mm = minmod(l, r);
     return zero ? cst<T>(0.0) : first ? r : last ? l : mm;
@@ -708,7 +709,7 @@ template <class T, int TW, int TH> struct Stage {
     }
   }
   static HF_HD T slope_x_eta(const S& s, const StageArgs<T>& a, int li, int lj, int64_t gj) {
-    if constexpr (kSelectSlopes) {
+    if constexpr (kSelectEta) {
       const bool zero = (a.nc < 2) | ((s.nd[S::i3(li, lj)] & 1) != 0);
       return pick_slope(zero, gj == 0, gj + 1 == a.nc, s.dxe[S::idx_(li, lj)], s.dxe[S::idx_(li, lj + 1)]);
     }
@@ -750,7 +751,7 @@ template <class T, int TW, int TH> struct Stage {
     }
   }
   static HF_HD T slope_y_eta(const S& s, const StageArgs<T>& a, int li, int lj, int64_t gi) {
-    if constexpr (kSelectSlopes) {
+    if constexpr (kSelectEta) {
       const bool zero = (a.nr < 2) | ((s.nd[S::i3(li, lj)] & 2) != 0);
       return pick_slope(zero, gi == 0, gi + 1 == a.nr, s.dye[S::idy_(li, lj)], s.dye[S::idy_(li + 1, lj)]);
     }
