@@ -128,9 +128,6 @@ hf2d_config default_config() {
 
 // Arguments for a launch reading state buffer `in`, writing buffer `out`,
 // with U^n in buffer `un`.
-// The active-tile list is used when it cannot change a bit (hf2d_tile.h,
-// tile_skipped): f64/f32 state, SSP-RK2, no rain, no friction on dry
-// cells, not switched off.
 // Grids of at most this many tiles run every tile: they fit the device in
 // one or two rounds of workgroups, so tiles left out save no time, while the
 // list costs a compaction launch per step and a dependent load of the list
@@ -139,6 +136,10 @@ hf2d_config default_config() {
 #define HF_LIST_MIN_TILES 512
 #endif
 
+// The active-tile list is used when it cannot change a bit (hf2d_tile.h,
+// tile_skipped): f64/f32 state, SSP-RK2, no rain, no friction on dry
+// cells, not switched off; and where it can save time: more than
+// HF_LIST_MIN_TILES tiles, unless HF2D_OPT_ACTIVE_TILES_ANY_SIZE is set.
 bool list_usable(const hf2d_sim* s) {
   const bool big = (int64_t)s->tiles_x * s->tiles_y > HF_LIST_MIN_TILES ||
                    (s->cfg.options & HF2D_OPT_ACTIVE_TILES_ANY_SIZE) != 0;
