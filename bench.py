@@ -522,7 +522,7 @@ def main():
     ap.add_argument("--serve-rates", type=float, nargs="+", default=(4.0, 16.0, float("inf")))
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+        print(f"bench.py: the step runs on one GPU; --gpus {args.gpus} is not supported "
               "(a single grid would shard by row strips with halo exchange, scenarios by footprint)",
               file=sys.stderr)
         sys.exit(2)
