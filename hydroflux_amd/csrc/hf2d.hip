@@ -92,7 +92,12 @@ void state_changed(hf2d_sim* s) {
   s->driver_async = false;
 }
 
+// Destroy the captured step graph. Launches of it may still be in flight
+// (hf2d_enqueue_steps returns without waiting, and so do the setters that
+// call this), and HIP does not defer the destruction of an executable graph
+// that is still running, so the stream is drained first.
 void drop_graph(hf2d_sim* s) {
+  if (s->gexec && s->stream) (void)hipStreamSynchronize(s->stream);
   if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
   if (s->graph) (void)hipGraphDestroy(s->graph);
   s->gexec = nullptr;
@@ -407,7 +412,8 @@ template <class T> int enqueue_compact(hf2d_sim* s) {
     if (rc) return rc;
   }
   hipLaunchKernelGGL(compact_kernel, dim3(1), dim3(1024), 0, s->stream, s->tile_wet, s->tile_pin, s->tile_run,
-                     s->tile_act, s->tiles_x, s->tiles_y, s->tile_list, s->tile_list + nt);
+                     s->tile_act, s->tiles_x, s->tiles_y, s->tile_list, s->tile_list + nt, s->ctrl, s->cfg.cfl,
+                     s->dx, s->dy, s->cfg.max_dt);
   HIPCHK(s, hipGetLastError());
   return HF2D_OK;
 }

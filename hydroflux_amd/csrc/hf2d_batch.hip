@@ -275,7 +275,11 @@ template <class T> StageArgs<T> scen_args(hf2d_batch* b, int slot, int in, int o
   return a;
 }
 
+// Launches of the graph may still be in flight (hf2d_batch_step returns
+// without waiting); the stream is drained before the executable graph is
+// destroyed.
 void destroy_graph(hf2d_batch* b) {
+  if (b->gexec && b->stream) (void)hipStreamSynchronize(b->stream);
   if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
   b->gexec = nullptr;

@@ -259,11 +259,16 @@ __global__ __launch_bounds__(NT) void stage_kernel(StageArgs<T> a) {
 
 // Rebuild the active-tile list for the step about to run (one workgroup of
 // 1024 threads): tiles in row-major order without the ones tile_skipped()
-// leaves out; *n_work = its length. act: per-tile scratch.
+// leaves out; *n_work = its length. act: per-tile scratch. A step that will
+// not run (hf::step_will_run on the driver's control block c with the first
+// stage's cfl, dx, dy, max_dt) leaves the counts and the list as they are:
+// its stage kernels return in their prologue without reading the list.
 __global__ __launch_bounds__(1024) void compact_kernel(const unsigned char* __restrict__ wet,
                                                        const unsigned char* __restrict__ pinned,
                                                        unsigned char* quiet_run, unsigned char* act, int tiles_x,
-                                                       int tiles_y, int* list, int* n_work) {
+                                                       int tiles_y, int* list, int* n_work, const Ctrl* c,
+                                                       double cfl, double dx, double dy, double max_dt) {
+  if (!hf::step_will_run(c, cfl, dx, dy, max_dt)) return;  // the same decision in every thread
   __shared__ int counts[1024];
   const int n = tiles_x * tiles_y;
   const int per = (n + 1023) / 1024;

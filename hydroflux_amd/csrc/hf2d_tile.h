@@ -1311,6 +1311,23 @@ HF_HD double dt_from_ctrl(double sx, double sy, double cap, double cfl, double d
   return dt;
 }
 
+// Whether the driver step about to start runs: the decisions of the first
+// stage's prologue (stage_prologue, sim.rs:209-240) on the control block
+// as that prologue will read it, without its side effects. The active-tile
+// compaction runs before the first stage and uses it, so that steps that
+// turn out to be no-ops (the clock reached its target, the budget is spent,
+// a failed or finished driver, the trailing steps of a captured chunk) do
+// not advance the tiles' quiet-step counts.
+HF_HD bool step_will_run(const Ctrl* c, double cfl, double dx, double dy, double max_dt) {
+  if ((c->status | c->done) != 0) return false;
+  if (c->use_target && !(c->t < c->t_target)) return false;
+  if (c->budget == 0) return false;
+  const int p = c->parity;
+  const double sx = hfm::as_f64(c->ws[p][0]), sy = hfm::as_f64(c->ws[p][1]);
+  const double dt = dt_from_ctrl(sx, sy, c->use_target ? (c->t_target - c->t) : c->dt_cap, cfl, dx, dy, max_dt);
+  return __builtin_isfinite(dt) && dt > 0.0;
+}
+
 // Forcing-series values at the end of the step about to run (t + dt), read
 // by the final stage's forcing.
 HF_HD void publish_series(Ctrl* c, const SeriesDesc* s, double dt) {

@@ -45,6 +45,9 @@ struct Emu {
   // active-tile list, as the device driver keeps it
   int list_on = 1;
   bool list_valid = false;
+  // run_until in chunks with trailing no-op steps, as the device driver
+  // (0: start no step that will not run)
+  int chunked = 1;
   std::vector<unsigned char> wet, run, pin;
   std::vector<int> list;
 };
@@ -219,6 +222,13 @@ template <class T, int TW, int TH> bool build_list(Emu& e, StageArgs<T>& a, Stag
       if (!(e.st[e.cur][0][(size_t)i] <= H_DRY)) e.wet[(size_t)((i / e.nc) / TH * tx + (i % e.nc) / TW)] = 1;
     e.list_valid = true;
   }
+  // compact_kernel: a step that will not run leaves counts and list alone
+  if (!step_will_run(&e.ctrl, e.cfl, e.dx, e.dy, e.max_dt)) {
+    *n_work = (int64_t)e.list.size();
+    a.tile_list = b.tile_list = e.list.data();
+    b.tile_wet = e.wet.data();
+    return true;
+  }
   e.list.clear();
   for (int y = 0; y < ty; ++y)
     for (int x = 0; x < tx; ++x) {
@@ -283,7 +293,25 @@ template <class T, int TW, int TH> int run_driver(Emu& e, bool use_target, doubl
   c.dt_cap = __builtin_inf();
   c.budget = use_target ? e.max_steps : n;
   if (use_target) {
-    while (c.status == 0 && c.done == 0) driver_step<T, TW, TH>(e);
+    // hf2d_run_until enqueues chunks of 16, 32, ..., 256 steps and polls the
+    // clock between chunks: the rest of the chunk in which the clock reaches
+    // its target (or the driver fails) runs as no-op steps, as here.
+    // Unchunked: no step is started that will not run; the first stage's
+    // prologue alone records why the driver stopped.
+    int64_t chunk = 16;
+    for (;;) {
+      if (!e.chunked && !step_will_run(&c, e.cfl, e.dx, e.dy, e.max_dt)) {
+        double dt;
+        StageArgs<T> a = args<T>(e, e.cur, 1 - e.cur, e.cur, TW, TH);
+        a.dt_mode = 1;
+        a.first_stage = 1;
+        (void)stage_prologue(a, true, &dt);
+        break;
+      }
+      for (int64_t k = 0; k < (e.chunked ? chunk : 1); ++k) driver_step<T, TW, TH>(e);
+      if (c.status != 0 || c.done != 0) break;
+      if (chunk < 256) chunk *= 2;
+    }
   } else {
     for (int64_t k = 0; k < n && c.status == 0; ++k) driver_step<T, TW, TH>(e);
   }
@@ -437,6 +465,17 @@ void emu_configure(void* p, double cfl, int integrator, int with_bcs, double max
 
 void emu_set_active_tiles(void* p, int on) { static_cast<Emu*>(p)->list_on = on; }
 int64_t emu_tiles_skipped() { return g_tiles_skipped; }
+
+void emu_set_chunked(void* p, int on) { static_cast<Emu*>(p)->chunked = on; }
+
+// The active-tile list's per-tile quiet-step counts (up to cap entries);
+// returns their number (0 while the list is not in use).
+int64_t emu_tile_runs(void* p, unsigned char* out, int64_t cap) {
+  const Emu& e = *static_cast<Emu*>(p);
+  const int64_t n = e.list_valid ? (int64_t)e.run.size() : 0;
+  for (int64_t k = 0; k < n && k < cap; ++k) out[k] = e.run[(size_t)k];
+  return n;
+}
 
 void emu_ssprk2_step(void* p, double dt) { dispatch(*static_cast<Emu*>(p), 0, dt, 0, false); }
 void emu_forward_euler_step(void* p, double dt) { dispatch(*static_cast<Emu*>(p), 1, dt, 0, false); }

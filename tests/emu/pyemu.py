@@ -47,6 +47,9 @@ def lib(asan: bool = False):
         L.emu_set_active_tiles.argtypes = [C.c_void_p, C.c_int]
         L.emu_tiles_skipped.restype = C.c_int64
         L.emu_tiles_skipped.argtypes = []
+        L.emu_set_chunked.argtypes = [C.c_void_p, C.c_int]
+        L.emu_tile_runs.restype = C.c_int64
+        L.emu_tile_runs.argtypes = [C.c_void_p, C.POINTER(C.c_ubyte), C.c_int64]
         _LIBS[name] = L
     return _LIBS[name]
 
@@ -72,6 +75,17 @@ class EmuSim:
 
     def tiles_skipped(self) -> int:
         return int(self._L.emu_tiles_skipped())
+
+    def set_chunked(self, on: bool):
+        """run_until in the device driver's chunks of steps (trailing no-op
+        steps after the target) or stopping at the target."""
+        self._L.emu_set_chunked(self._h, 1 if on else 0)
+
+    def tile_runs(self) -> np.ndarray:
+        """Per-tile counts of consecutive quiet steps of the active-tile list."""
+        buf = np.zeros(1 << 16, np.uint8)
+        n = int(self._L.emu_tile_runs(self._h, buf.ctypes.data_as(C.POINTER(C.c_ubyte)), buf.size))
+        return buf[:n].copy()
 
     def close(self):
         if self._h:

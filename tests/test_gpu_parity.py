@@ -444,3 +444,41 @@ def test_errors_through_the_abi():
     g.configure(max_dt=0.01)
     g.run_steps(2)
     assert g.time()[1] == 2
+
+
+@pytest.mark.parametrize("change", ["set_boundaries", "configure_dual", "kernel_timing", "gauges"])
+def test_graph_dropped_while_enqueued_steps_run(change):
+    """hf2d_enqueue_steps returns with graph replays in flight; a call that
+    drops the captured graph right after it (new boundaries, a Dual
+    reconfiguration without sources, kernel timing, gauges) must not
+    destroy it under the running replays. Results stay the oracle's bits."""
+    _say(f"graph dropped with replays in flight: {change}")
+    dual = change == "configure_dual"
+    c = cases.bumpy_wet_dry(384, 448, 17)
+    kw = dict(manning_tangent=c.manning) if dual else {}
+    g = c.load(_hf().Grid2D(c.nr, c.nc, c.dx, c.dy, dual=dual), **kw)
+    o = c.load(OracleSim(c.nr, c.nc, c.dx, c.dy, dual=dual), **kw)
+    for s in (g, o):
+        s.configure(cfl=0.4, max_dt=0.05, rain_rate=2e-4)
+    g.enqueue_steps(64)
+    o.run_steps(64)
+    if change == "set_boundaries":
+        for s in (g, o):
+            s.set_boundaries([cases.WALL, cases.TRANS, cases.WALL, cases.TRANS])
+    elif change == "configure_dual":
+        for s in (g, o):
+            s.configure(cfl=0.35, max_dt=0.05, rain_rate=2e-4)
+    elif change == "kernel_timing":
+        g.set_kernel_timing(True)
+    else:
+        g.set_gauges([(10, 10), (200, 300)], capacity=64)
+    g.synchronize()
+    for s in (g, o):
+        s.run_steps(40)
+    assert g.time() == o.time()
+    if dual:
+        (gv, gt), (ov, ot) = g.get_state(), o.get_state()
+        assert np.array_equal(gv, ov) and np.array_equal(gt, ot, equal_nan=True)
+    else:
+        assert np.array_equal(g.get_state(), o.get_state())
+    g.close()

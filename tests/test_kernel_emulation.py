@@ -161,3 +161,29 @@ def test_active_tile_list_dual_runs_every_tile():
     ev, et = e.get_state()
     # the Discharge-on-dry ghost gives NaN tangents along the inflow edge in both
     assert np.array_equal(ov, ev) and np.array_equal(ot, et, equal_nan=True)
+
+
+@pytest.mark.parametrize("tile", [(8, 4), (16, 8)], ids=lambda t: f"{t[0]}x{t[1]}")
+def test_active_tile_list_run_until_split_across_calls(tile):
+    """run_until split over several calls: the no-op steps that end each
+    call's last chunk of device steps (the clock has reached its target) do
+    not count as quiet steps of the active-tile list, so its counts equal
+    those of a driver that starts no step that will not run. The first call
+    ends after one step, right after the counts were reset. Bounds-checked
+    tile code; results are the oracle's bits."""
+    c = cases.mostly_dry(60, 84, 5, west=cases.WALL)
+    src = [(c.nr - 6, 8, 0.02)]
+    chunked = c.load(EmuSim(c.nr, c.nc, c.dx, c.dy, tile=tile))
+    exact = c.load(EmuSim(c.nr, c.nc, c.dx, c.dy, tile=tile))
+    exact.set_chunked(False)
+    o = c.load(OracleSim(c.nr, c.nc, c.dx, c.dy))
+    for s in (chunked, exact, o):
+        s.configure(cfl=0.4, max_dt=0.2, sources=src)
+    for t in (0.01, 0.9, 1.7, 2.2, 3.0, 3.05, 4.0):
+        for s in (chunked, exact, o):
+            assert s.run_until(t) == 0
+        assert np.array_equal(chunked.tile_runs(), exact.tile_runs())
+    assert chunked.tile_runs().max() == 2  # the list was in use, with settled tiles
+    assert chunked.time() == exact.time() == o.time()
+    assert np.array_equal(chunked.get_state(), o.get_state())
+    assert np.array_equal(exact.get_state(), o.get_state())

@@ -144,3 +144,56 @@ def test_mesh_validation_mirrors_reference_panics():
         Mesh2D(np.zeros((2, 2)), 1.0, 1.0, np.zeros((3, 2)))
     with pytest.raises(InvalidConfig):
         Mesh2D(np.zeros((0, 2)), 1.0, 1.0, 0.0)
+
+
+def _header_functions():
+    """{name: [(param name, C type)], return type} of every entry point the
+    header declares."""
+    txt = re.sub(r"/\*.*?\*/", "", open(os.path.join(ROOT, "include", "hydroflux_amd.h")).read(), flags=re.S)
+    out = {}
+    for ret, name, params in re.findall(r"^\s*((?:const\s+)?\w+\s*\*?)\s*(hf2d_\w+)\s*\(([^;]*?)\)\s*;", txt,
+                                        flags=re.M | re.S):
+        ps = []
+        for p in (q for q in params.split(",") if q.strip()):
+            p = " ".join(p.split())
+            arr = re.search(r"\[\d+\]$", p) is not None
+            p = re.sub(r"\[\d+\]$", "", p)
+            const = p.startswith("const ")
+            base, pname = p[6:].replace("*", " ").split() if const else p.replace("*", " ").split()
+            ps.append((pname, (const, base, p.count("*") + (1 if arr else 0))))
+        out[name] = (ps, " ".join(ret.split()))
+    return out
+
+
+def _rust_type(const, base, stars):
+    t = {"int32_t": "i32", "int64_t": "i64", "double": "f64", "int": "c_int"}.get(base, base)
+    for k in range(stars):
+        t = ("*const " if const and k == 0 else "*mut ") + t
+    return t
+
+
+def test_integration_rust_block_declares_every_entry_point():
+    """INTEGRATION.md's hydroflux-hip-sys block declares every function of
+    include/hydroflux_amd.h, with the header's parameter names, the Rust
+    types of the C types, and the config struct's fields in order."""
+    md = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    block = md[md.index('extern "C" {'):]
+    block = block[:block.index("\n}\n")]
+    rust = {}
+    for name, params, ret in re.findall(r"pub fn (hf2d_\w+)\(([^)]*)\)\s*->\s*([^;]+);", block, flags=re.S):
+        ps = [tuple(x.strip() for x in p.split(":", 1)) for p in params.split(",") if p.strip()]
+        rust[name] = (ps, " ".join(ret.split()))
+    header = _header_functions()
+    assert set(rust) == set(header), set(rust) ^ set(header)
+    ret_map = {"int": "c_int", "const char*": "*const c_char"}
+    for name, (ps, ret) in header.items():
+        rps, rret = rust[name]
+        assert rret == ret_map[ret], name
+        assert [p[0] for p in rps] == [p[0] for p in ps], name
+        assert [p[1] for p in rps] == [_rust_type(*t) for _, t in ps], name
+    # struct hf2d_config field order and names
+    htxt = open(os.path.join(ROOT, "include", "hydroflux_amd.h")).read()
+    hcfg = re.search(r"typedef struct hf2d_config \{(.*?)\} hf2d_config;", htxt, flags=re.S).group(1)
+    hfields = re.findall(r"^\s*\w+\s+(\w+);", hcfg, flags=re.M)
+    rcfg = re.search(r"pub struct hf2d_config \{(.*?)\}", md, flags=re.S).group(1)
+    assert re.findall(r"pub (\w+):", rcfg) == hfields
