@@ -5,6 +5,16 @@
   in Dual seeded on a global Manning multiplier: the fused driver for 200
   steps, bit for bit with the oracle (its OpenMP rows are the serial
   restatement's bits), clock and step count included.
+- 4096 x 4096, the same construction with the bed raised 2000 m: the FP32
+  precision form against the FP64 run at the same simulated time, depth
+  L1 / Linf / peak / extent IoU (h > 1 cm) / mass, with the Linf bound tied
+  to the FP64 run's own sensitivity to a 1e-9 m bed perturbation.
+- the 256-scenario mix (sides log-uniform in [128, 2048], aspect U(1, 4),
+  98.7 M cells) resident in one FP64 batch and one FP32 batch: every clock
+  advances, every state is finite, and scenarios chosen by size and shape
+  equal single-grid runs bit for bit.
+- serving on requests drawn from that mix, the 2048-side grids included,
+  25 % Dual: each result equals the same request run alone.
 """
 import sys
 
@@ -65,3 +75,154 @@ def test_flood_1024_bitwise(kind):
         assert np.array_equal(gv, o.get_state())
     assert np.count_nonzero(gv[..., 0] > 1e-6) > 0.9 * gv[..., 0].size  # rain wets the domain
     g.close()
+
+
+def _metrics(test, ref, dx, vol_test, vol_ref):
+    from hydroflux_amd.metrics import depth_metrics
+
+    return depth_metrics(test, ref, dx, dx, vol_test=vol_test, vol_ref=vol_ref)
+
+
+def test_precision_form_4096_against_fp64():
+    from hydroflux_amd import Grid2D
+    from hydroflux_amd.scenarios import flood_scenario
+
+    _say("precision form 4096x4096 +2000 m: FP64 200 steps, FP32 and perturbed FP64 to the same time")
+    sc = flood_scenario(n=4096, dx=5.0, offset=2000.0)
+
+    def grid(kind, bed):
+        g = Grid2D(sc.n_rows, sc.n_cols, sc.dx, sc.dx, kind=kind)
+        g.set_mesh(bed, sc.manning)
+        g.set_state(sc.dry_state())
+        g.set_boundaries(list(sc.bc_kinds))
+        g.configure(cfl=sc.cfl, max_dt=sc.max_dt, friction=True, friction_dry_tol=1e-6, rain_rate=sc.rain_rate,
+                    sources=sc.sources)
+        return g
+
+    g64 = grid("f64", sc.bed)
+    g64.run_steps(200)
+    t_ref, _ = g64.time()
+    h64 = g64.get_state()[..., 0]
+    v64 = g64.total_volume()
+    g64.close()
+    g32 = grid("f32", sc.bed)
+    g32.run_until(t_ref)
+    assert abs(g32.time()[0] - t_ref) <= 1e-9 * t_ref
+    h32 = g32.get_state()[..., 0]
+    m32 = _metrics(h32, h64, sc.dx, g32.total_volume(), v64)
+    g32.close()
+    rng = np.random.default_rng(12345)
+    gp = grid("f64", sc.bed + 1e-9 * rng.standard_normal(sc.bed.shape))
+    gp.run_until(t_ref)
+    mp = _metrics(gp.get_state()[..., 0], h64, sc.dx, gp.total_volume(), v64)
+    gp.close()
+    _say(f"  t = {t_ref:.2f} s; FP32: {m32}")
+    _say(f"  FP64, bed + 1e-9 m noise: {mp}")
+    assert np.all(np.isfinite(h32))
+    assert m32["depth_L1_m"] < 5e-5
+    assert abs(m32["mass_balance_rel_err"]) < 1e-5
+    # 200 steps in, only a few hundred cells hold more than 1 cm (the valley
+    # and the inflow): one cell more or less moves the IoU by ~0.3 %, in the
+    # FP64 run's own sensitivity too
+    assert m32["extent_iou_1cm"] > 0.9 and m32["extent_iou_1cm"] >= mp["extent_iou_1cm"] - 0.05
+    assert abs(m32["peak_depth_err_m"]) < 1e-3
+    assert m32["depth_Linf_m"] <= 3.0 * mp["depth_Linf_m"] + 1e-3
+
+
+def _mix_specs(mix, t_end):
+    from hydroflux_amd.batch import ScenarioSpec
+
+    return [ScenarioSpec(bed=s.bed, manning=s.manning, dx=s.dx, bc_kinds=s.bc_kinds, rain_rate=s.rain_rate,
+                         sources=s.sources, t_end=t_end, max_dt=s.max_dt, cfl=s.cfl) for s in mix]
+
+
+def _picked(mix):
+    """Scenario indices: smallest, largest, most elongated, squarest and
+    four more in list order."""
+    cells = [s.n_cells for s in mix]
+    aspect = [max(s.n_rows, s.n_cols) / min(s.n_rows, s.n_cols) for s in mix]
+    pick = [int(np.argmin(cells)), int(np.argmax(cells)), int(np.argmax(aspect)), int(np.argmin(aspect))]
+    for k in range(len(mix)):
+        if len(pick) == 8:
+            break
+        if k not in pick:
+            pick.append(k)
+    return pick
+
+
+@pytest.mark.parametrize("kind", ["f64", "f32"])
+def test_batch_256_scenario_mix(kind):
+    from hydroflux_amd import Grid2D
+    from hydroflux_amd.batch import Batch
+    from hydroflux_amd.scenarios import scenario_mix
+
+    _say(f"batch {kind}: the 256-scenario mix, 32 steps")
+    mix = scenario_mix(256)
+    assert max(max(s.n_rows, s.n_cols) for s in mix) >= 1900
+    b = Batch(kind, capacity_bytes=24 << 30, max_slots=512)
+    slots = [b.add(sp) for sp in _mix_specs(mix, 1e9)]
+    b.step(32)
+    b.synchronize()
+    t, steps, status, done, used = b.poll()
+    for s in slots:
+        assert used[s] and status[s] == 0 and steps[s] == 32 and t[s] > 0.0
+    pick = _picked(mix)
+    for k, s in enumerate(slots):
+        out = b.download(s, mix[k].bed.shape)
+        assert np.all(np.isfinite(out)), k
+        if k not in pick:
+            continue
+        sc = mix[k]
+        g = Grid2D(sc.n_rows, sc.n_cols, sc.dx, sc.dx, kind=kind)
+        g.set_mesh(sc.bed, sc.manning)
+        g.set_state(sc.dry_state())
+        g.set_boundaries(list(sc.bc_kinds))
+        g.configure(cfl=sc.cfl, max_dt=sc.max_dt, rain_rate=sc.rain_rate, sources=sc.sources)
+        g.run_steps(32)
+        assert g.time() == (float(t[s]), 32), k
+        assert np.array_equal(g.get_state(), out), (k, sc.n_rows, sc.n_cols)
+        g.close()
+    b.close()
+
+
+def test_serving_on_the_mix_sizes():
+    from hydroflux_amd.batch import Batch, ScenarioSpec
+    from hydroflux_amd.scenarios import scenario_mix
+    from hydroflux_amd.serving import Request, Server, poisson_arrivals
+
+    _say("serving: 12 requests from the 256-scenario mix, 2048-side grids included")
+    mix = scenario_mix(256)
+    order = np.argsort([-s.n_cells for s in mix])
+    idx = list(order[:3]) + list(order[-3:]) + list(order[100:106])  # largest, smallest, middle
+    arr = poisson_arrivals(len(idx), rate=50.0, seed=9)
+    reqs = []
+    for k, (i, a) in enumerate(zip(idx, arr)):
+        s = mix[i]
+        dual = k % 4 == 3
+        sp = ScenarioSpec(bed=s.bed, manning=s.manning, dx=s.dx, bc_kinds=s.bc_kinds, rain_rate=s.rain_rate,
+                          sources=s.sources, t_end=float(4 + k), max_dt=s.max_dt, cfl=s.cfl,
+                          manning_tangent=((s.land_class == 2).astype(np.float64) if dual else None))
+        reqs.append(Request(k, float(a), sp, "dual" if dual else "f32"))
+    srv = Server({"f32": 8 << 30, "dual": 4 << 30}, max_slots=64, chunk_steps=16, tile_config={"f32": 0})
+    rep = srv.run(reqs)
+    srv.close()
+    summ = rep.summary()
+    _say(f"  {summ}")
+    assert summ["failed"] == 0 and np.all(np.isfinite(rep.latencies))
+    assert max(max(r.spec.shape) for r in reqs) >= 1900
+    for r in reqs:
+        assert r.sim_time == r.spec.t_end
+        b = Batch(r.kind, capacity_bytes=4 << 30, max_slots=4)
+        b.set_config(0)
+        s = b.add(r.spec)
+        for _ in range(1000):
+            b.step(16)
+            b.synchronize()
+            if b.poll()[3][s]:
+                break
+        out = b.download(s, r.spec.shape)
+        b.close()
+        if r.kind == "dual":
+            assert np.array_equal(out[0], r.result) and np.array_equal(out[1], r.tangent), r.rid
+        else:
+            assert np.array_equal(out, r.result), r.rid
