@@ -65,6 +65,7 @@ SIGNATURES = {
     "hf2d_set_kernel_timing": (C.c_int, [C.c_void_p, C.c_int32]),
     "hf2d_kernel_timing": (C.c_int, [C.c_void_p, _D, _I64]),
     "hf2d_total_volume": (C.c_int, [C.c_void_p, _D]),
+    "hf2d_total_volume_noalloc": (C.c_int, [C.c_void_p, _D]),
     "hf2d_copy_probe": (C.c_int, [C.c_int64, C.c_int32, _D]),
     "hf2d_math_probe": (C.c_int, [_D, _D, C.c_int64, _D]),
     "hf2d_batch_create": (C.c_int, [C.c_int32, C.c_int64, C.c_int32, C.POINTER(C.c_void_p)]),

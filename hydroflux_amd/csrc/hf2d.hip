@@ -12,6 +12,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -22,6 +23,7 @@
 // ============================================================== handle
 struct hf2d_sim {
   int64_t nr = 0, nc = 0;
+  int device = 0;  // the HIP device current at creation: its buffers and stream
   double dx = 1.0, dy = 1.0;
   int kind = HF2D_F64;
   int n_tan = 0;  // tangents per value: Dual 1, DualN<N> N, else 0
@@ -731,6 +733,10 @@ int hf2d_create(int64_t n_rows, int64_t n_cols, double dx, double dy, int32_t sc
     return HF2D_INVALID_ARGUMENT;
   if (!(dx > 0.0) || !(dy > 0.0) || n_rows <= 0 || n_cols <= 0) return HF2D_INVALID_CONFIG;
   hf2d_sim* s = new hf2d_sim();
+  if (hipGetDevice(&s->device) != hipSuccess) {
+    delete s;
+    return HF2D_HIP_ERROR;
+  }
   s->nr = n_rows;
   s->nc = n_cols;
   s->dx = dx;
@@ -943,34 +949,94 @@ int hf2d_download_state(hf2d_sim* s, double* aos, double* tan_aos) {
   return HF2D_OK;
 }
 
-int hf2d_total_volume(hf2d_sim* s, double* volume) {
-  if (!valid(s) || !volume) return fail(s, HF2D_INVALID_ARGUMENT, "null argument");
+namespace {
+constexpr int VOL_BLOCKS = 1024;  // volume_kernel workgroups: one (sum, compensation) pair each
+
+// Enqueue the volume reduction of the current state into part (2 *
+// VOL_BLOCKS doubles on the handle's device).
+hipError_t enqueue_volume(hf2d_sim* s, double* part) {
   const int64_t n = s->nr * s->nc;
-  const int nb = 1024;
-  double* part = nullptr;
-  HIPCHK(s, hipMalloc(&part, 2 * nb * sizeof(double)));
-  if (s->n_tan > 0)  // the value plane alone
-    hipLaunchKernelGGL(volume_kernel<double>, dim3(nb), dim3(256), 0, s->stream,
+  if (s->kind == HF2D_F32)
+    hipLaunchKernelGGL(volume_kernel<float>, dim3(VOL_BLOCKS), dim3(256), 0, s->stream,
+                       fld<float>(s->st[s->cur][0], nullptr), n, part);
+  else  // f64, and the value plane alone of the Dual kinds
+    hipLaunchKernelGGL(volume_kernel<double>, dim3(VOL_BLOCKS), dim3(256), 0, s->stream,
                        fld<double>(s->st[s->cur][0], nullptr), n, part);
-  else if (s->kind == HF2D_F32)
-    hipLaunchKernelGGL(volume_kernel<float>, dim3(nb), dim3(256), 0, s->stream, fld<float>(s->st[s->cur][0], nullptr),
-                       n, part);
-  else
-    hipLaunchKernelGGL(volume_kernel<double>, dim3(nb), dim3(256), 0, s->stream,
-                       fld<double>(s->st[s->cur][0], nullptr), n, part);
-  std::vector<double> hp(2 * nb);
-  hipError_t e = hipGetLastError();
-  if (e == hipSuccess) e = hipMemcpyAsync(hp.data(), part, 2 * nb * sizeof(double), hipMemcpyDeviceToHost, s->stream);
-  if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
-  (void)hipFree(part);
-  if (e != hipSuccess) return fail(s, HF2D_HIP_ERROR, hipGetErrorString(e));
-  double sum = 0.0, comp = 0.0;  // Neumaier over the block pairs, fixed order
-  for (int k = 0; k < 2 * nb; ++k) {
+  return hipGetLastError();
+}
+
+// Neumaier sum of the block pairs in a fixed order, times the cell area.
+double fold_volume(const hf2d_sim* s, const double* hp) {
+  double sum = 0.0, comp = 0.0;
+  for (int k = 0; k < 2 * VOL_BLOCKS; ++k) {
     const double x = hp[k], t = sum + x;
     comp += (__builtin_fabs(sum) >= __builtin_fabs(x)) ? (sum - t) + x : (x - t) + sum;
     sum = t;
   }
-  *volume = (sum + comp) * s->dx * s->dy;
+  return (sum + comp) * s->dx * s->dy;
+}
+
+}  // namespace
+
+// Scratch of hf2d_total_volume_noalloc: the per-workgroup partial sums, in a
+// file-scope device array (16 KiB, whatever the grid's size) instead of a
+// buffer allocated per call. Each device holds its own copy until the
+// process ends. In host code `scratch` names a host-side stand-in, not the
+// device array: its device address is looked up once per device
+// (scratch_on) and only that address is used.
+static __device__ __align__(256) double scratch[2 * 1024];
+
+namespace {
+static_assert(sizeof(scratch) == 2 * VOL_BLOCKS * sizeof(double), "one (sum, compensation) pair per workgroup");
+constexpr int MAX_DEVICES = 64;
+double* g_scratch_ptr[MAX_DEVICES] = {};
+std::mutex g_scratch_mu;
+
+// Device address of `scratch` on the given device (looked up with that
+// device current, then the caller's current device is restored).
+hipError_t scratch_on(int device, double** out) {
+  if (device < 0 || device >= MAX_DEVICES) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    int cur = 0;
+    hipError_t e = hipGetDevice(&cur);
+    if (e != hipSuccess) return e;
+    if (cur != device && (e = hipSetDevice(device)) != hipSuccess) return e;
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != device) (void)hipSetDevice(cur);
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[device] = static_cast<double*>(p);
+  }
+  *out = g_scratch_ptr[device];
+  return hipSuccess;
+}
+}  // namespace
+
+int hf2d_total_volume(hf2d_sim* s, double* volume) {
+  if (!valid(s) || !volume) return fail(s, HF2D_INVALID_ARGUMENT, "null argument");
+  double* part = nullptr;
+  HIPCHK(s, hipMalloc(&part, 2 * VOL_BLOCKS * sizeof(double)));
+  std::vector<double> hp(2 * VOL_BLOCKS);
+  hipError_t e = enqueue_volume(s, part);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(hp.data(), part, 2 * VOL_BLOCKS * sizeof(double), hipMemcpyDeviceToHost, s->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+  (void)hipFree(part);
+  if (e != hipSuccess) return fail(s, HF2D_HIP_ERROR, hipGetErrorString(e));
+  *volume = fold_volume(s, hp.data());
+  return HF2D_OK;
+}
+
+int hf2d_total_volume_noalloc(hf2d_sim* s, double* volume) {
+  if (!valid(s) || !volume) return fail(s, HF2D_INVALID_ARGUMENT, "null argument");
+  double* part = nullptr;
+  HIPCHK(s, scratch_on(s->device, &part));
+  double hp[2 * VOL_BLOCKS];  // 16 KiB on the caller's stack, no heap
+  HIPCHK(s, enqueue_volume(s, part));
+  HIPCHK(s, hipMemcpyAsync(hp, part, sizeof hp, hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(s, hipStreamSynchronize(s->stream));
+  *volume = fold_volume(s, hp);
   return HF2D_OK;
 }
 

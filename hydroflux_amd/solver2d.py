@@ -398,10 +398,13 @@ class Grid2D:
             _check(lib().hf2d_read_gauges(self._h, n.value, _p(buf), C.byref(n)), self._h)
         return buf[:, 0].copy(), buf[:, 1:].copy()
 
-    def total_volume(self) -> float:
-        """sum(h) dx dy of the current state (FP64, compensated)."""
+    def total_volume(self, noalloc: bool = False) -> float:
+        """sum(h) dx dy of the current state (FP64, compensated). noalloc:
+        through hf2d_total_volume_noalloc (library-owned scratch, no device
+        allocation; same bits)."""
         v = C.c_double()
-        _check(lib().hf2d_total_volume(self._h, C.byref(v)), self._h)
+        fn = lib().hf2d_total_volume_noalloc if noalloc else lib().hf2d_total_volume
+        _check(fn(self._h, C.byref(v)), self._h)
         return v.value
 
     def set_kernel_timing(self, on: bool):

@@ -220,6 +220,16 @@ int hf2d_kernel_timing(hf2d_sim* sim, double stage_ms[2], int64_t* count);
  * update.rs:1689-1717, computed on the device). */
 int hf2d_total_volume(hf2d_sim* sim, double* volume);
 
+/* hf2d_total_volume without allocating: no counterpart in the reference.
+ * For callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc): the same
+ * reduction and the same result bits, with its per-workgroup partial sums in
+ * a 16 KiB library-owned device array (one per GPU, kept until the process
+ * ends, whatever the grid's size) instead of a buffer allocated per call.
+ * Calls on one GPU share that array: calls that overlap on one GPU (two
+ * host threads, two handles on one device) race, so serialise them. */
+int hf2d_total_volume_noalloc(hf2d_sim* sim, double* volume);
+
 /* Bandwidth reference: average duration of a plain device copy of n_bytes
  * (read n_bytes, write n_bytes) over `reps` launches. Also the calibration
  * kernel for hardware traffic counters. */
