@@ -242,6 +242,8 @@ __global__ __launch_bounds__(NT) void stage_kernel(StageArgs<T> a) {
   __syncthreads();
   St::phase_diffs(s, a, gi0, gj0, tid, NT);
   __syncthreads();
+  St::phase_slopes(s, a, gi0, gj0, tid, NT);
+  __syncthreads();
   St::phase_faces(s, a, gi0, gj0, tid, NT);
   __syncthreads();
   St::phase_alpha(s, a, gi0, gj0, tid, NT, dt);
@@ -531,7 +533,8 @@ __global__ void math_probe_kernel(const double* x, const double* y, int64_t n, d
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   double xi = x[i];
-  out[4 * i + 0] = __builtin_sqrt(xi);
+  // the kernels' sqrt: the unscaled sequence from 2^-767 up, else IEEE sqrt
+  out[4 * i + 0] = xi >= 0x1p-767 ? hf::sqrt_ge_tiny(xi) : __builtin_sqrt(xi);
   out[4 * i + 1] = xi / y[i];
   out[4 * i + 2] = hfm::pow_4_3(xi);
   out[4 * i + 3] = hfm::pow_1_3m(xi);
@@ -639,6 +642,8 @@ __global__ __launch_bounds__(NT) void batch_stage_kernel(const StageArgs<T>* __r
     St::template phase_load<NT>(s, a, gi0, gj0, tid);
     __syncthreads();
     St::phase_diffs(s, a, gi0, gj0, tid, NT);
+    __syncthreads();
+    St::phase_slopes(s, a, gi0, gj0, tid, NT);
     __syncthreads();
     St::phase_faces(s, a, gi0, gj0, tid, NT);
     __syncthreads();

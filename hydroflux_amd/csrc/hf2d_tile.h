@@ -304,7 +304,12 @@ HF_HD void ghost_state(int kind, double param, int side, T h, T hu, T hv, T zc, 
   }
 }
 
-// riemann.rs:114-204
+// riemann.rs:114-204. The flux is the physical flux of one side K (the
+// left state when sl >= 0, the right when sr <= 0, else the side the contact
+// wave s* leaves upwind), plus the star-state correction when the wave fan
+// spans the face. Side K's operands are selected first and its flux is
+// evaluated once: the same expressions the reference evaluates on each of
+// its branches, one evaluation per face instead of one per branch.
 template <class T>
 HF_HD void hllc_normal_flux(T h_l, T qn_l, T qt_l, T h_r, T qn_r, T qt_r, T* fm, T* fn, T* ft) {
   using R = typename ScalarOf<T>::type;
@@ -317,8 +322,8 @@ HF_HD void hllc_normal_flux(T h_l, T qn_l, T qt_l, T h_r, T qn_r, T qt_r, T* fm,
     *ft = Z;
     return;
   }
-  T cl = lw ? rsqrt_(h_l * R(GRAVITY)) : Z;
-  T cr = rw ? rsqrt_(h_r * R(GRAVITY)) : Z;
+  T cl = lw ? rsqrt_wet(h_l * R(GRAVITY)) : Z;  // h > DRY_TOL: g h > 2^-767
+  T cr = rw ? rsqrt_wet(h_r * R(GRAVITY)) : Z;
   T un_l = lw ? div(qn_l, h_l) : Z;
   T un_r = rw ? div(qn_r, h_r) : Z;
   T ut_l = lw ? div(qt_l, h_l) : Z;
@@ -334,51 +339,35 @@ HF_HD void hllc_normal_flux(T h_l, T qn_l, T qt_l, T h_r, T qn_r, T qt_r, T* fm,
     sl = un_l - cl;
     sr = un_l + cl * R(2.0);
   }
-  if (val(sl) >= R(0)) {
-    if (lw) {
-      *fm = qn_l;
-      *fn = qn_l * un_l + h_l * h_l * R(0.5 * GRAVITY);
-      *ft = qn_l * ut_l;
-    } else {
-      *fm = Z;
-      *fn = Z;
-      *ft = Z;
-    }
-    return;
+  const bool exit_l = val(sl) >= R(0);
+  const bool star = !exit_l && !(val(sr) <= R(0));
+  T s_star = Z;
+  if (star) {
+    T denom = h_r * (un_r - sr) - h_l * (un_l - sl);
+    s_star = (rabs(val(denom)) > R(1.0e-12)) ? div(h_r * (un_r - sr) * sl - h_l * (un_l - sl) * sr, denom)
+                                             : (sl + sr) * R(0.5);
   }
-  if (val(sr) <= R(0)) {
-    if (rw) {
-      *fm = qn_r;
-      *fn = qn_r * un_r + h_r * h_r * R(0.5 * GRAVITY);
-      *ft = qn_r * ut_r;
-    } else {
-      *fm = Z;
-      *fn = Z;
-      *ft = Z;
-    }
-    return;
-  }
-  T denom = h_r * (un_r - sr) - h_l * (un_l - sl);
-  T s_star = (rabs(val(denom)) > R(1.0e-12))
-                 ? div(h_r * (un_r - sr) * sl - h_l * (un_l - sl) * sr, denom)
-                 : (sl + sr) * R(0.5);
-  // HLLC sampling (riemann.rs:191-203): the same expressions on the side
-  // the contact wave leaves upwind, with that side's operands selected
-  // first so a wavefront evaluates one star state instead of both branches.
-  const bool left = val(s_star) >= R(0);
+  const bool left = star ? val(s_star) >= R(0) : exit_l;
   const bool wK = left ? lw : rw;
   const T hK = left ? h_l : h_r;
   const T qnK = left ? qn_l : qn_r;
   const T qtK = left ? qt_l : qt_r;
   const T unK = left ? un_l : un_r;
   const T utK = left ? ut_l : ut_r;
-  const T sK = left ? sl : sr;
   T fmK = Z, fnK = Z, ftK = Z;
   if (wK) {
     fmK = qnK;
     fnK = qnK * unK + hK * hK * R(0.5 * GRAVITY);
     ftK = qnK * utK;
   }
+  if (!star) {
+    *fm = fmK;
+    *fn = fnK;
+    *ft = ftK;
+    return;
+  }
+  // HLLC star state on side K (riemann.rs:191-203)
+  const T sK = left ? sl : sr;
   T h_star = div(hK * (sK - unK), sK - s_star);
   *fm = fmK + (h_star - hK) * sK;
   *fn = fnK + (h_star * s_star - qnK) * sK;
@@ -429,11 +418,30 @@ template <class T, int TW, int TH> struct TileSmem {
   static constexpr int WFY = TW + 2, HFY = TH + 3, NFY = WFY * HFY;
   static constexpr int WA = TW + 2, HA = TH + 2, NA = WA * HA;
 
+  // minmod slopes of the cells whose faces the tile needs: along x for
+  // columns [-2, TW+1] in rows [-1, TH], along y for rows [-2, TH+1] in
+  // columns [-1, TW]
+  static constexpr int WSX = TW + 4, HSX = TH + 2, NSX = WSX * HSX;
+  static constexpr int WSY = TW + 2, HSY = TH + 4, NSY = WSY * HSY;
+
   T h[N3], z[N3], u[N3], v[N3];
-  T dxe[NDX], dxu[NDX], dxv[NDX];
-  T dye[NDY], dyu[NDY], dyv[NDY];
-  T fxm[NFX], fxa[NFX], fxb[NFX], fxt[NFX];
-  T fym[NFY], fya[NFY], fyb[NFY], fyt[NFY];
+  T sxe[NSX], sxu[NSX], sxv[NSX];
+  T sye[NSY], syu[NSY], syv[NSY];
+  // The face differences are read only to form the slopes, and the face
+  // fluxes are written after that: one block of LDS holds the one, then
+  // the other.
+  struct Diffs {
+    T dxe[NDX], dxu[NDX], dxv[NDX];
+    T dye[NDY], dyu[NDY], dyv[NDY];
+  };
+  struct Fluxes {
+    T fxm[NFX], fxa[NFX], fxb[NFX], fxt[NFX];
+    T fym[NFY], fya[NFY], fyb[NFY], fyt[NFY];
+  };
+  union {
+    Diffs d;
+    Fluxes f;
+  } w;
   T alpha[NA];
   unsigned char dry[N4];
   unsigned char nd[N3];  // bit 0: a dry cell within +-2 along x; bit 1: along y
@@ -454,6 +462,14 @@ template <class T, int TW, int TH> struct TileSmem {
   static HF_HD int idy_(int fi, int lj) {
     HF_BOUNDS_CHECK(fi >= -2 && fi < TH + 3 && lj >= -1 && lj < TW + 1);
     return (fi + 2) * WDY + (lj + 1);
+  }
+  static HF_HD int isx(int li, int lj) {
+    HF_BOUNDS_CHECK(li >= -1 && li < TH + 1 && lj >= -2 && lj < TW + 2);
+    return (li + 1) * WSX + (lj + 2);
+  }
+  static HF_HD int isy(int li, int lj) {
+    HF_BOUNDS_CHECK(li >= -2 && li < TH + 2 && lj >= -1 && lj < TW + 1);
+    return (li + 2) * WSY + (lj + 1);
   }
   static HF_HD int ifx(int li, int fj) {
     HF_BOUNDS_CHECK(li >= -1 && li < TH + 1 && fj >= -1 && fj < TW + 2);
@@ -624,9 +640,9 @@ template <class T, int TW, int TH> struct Stage {
           du = div(s.u[r] - s.u[l], dx);
           dv = div(s.v[r] - s.v[l], dx);
         }
-        s.dxe[k] = de;
-        s.dxu[k] = du;
-        s.dxv[k] = dv;
+        s.w.d.dxe[k] = de;
+        s.w.d.dxu[k] = du;
+        s.w.d.dxv[k] = dv;
       } else if (k < S::NDX + S::NDY) {
         const int kk = k - S::NDX;
         const int fi = kk / S::WDY - 2, lj = kk % S::WDY - 1;
@@ -641,9 +657,9 @@ template <class T, int TW, int TH> struct Stage {
           du = div(s.u[b] - s.u[t], dy);
           dv = div(s.v[b] - s.v[t], dy);
         }
-        s.dye[kk] = de;
-        s.dyu[kk] = du;
-        s.dyv[kk] = dv;
+        s.w.d.dye[kk] = de;
+        s.w.d.dyu[kk] = du;
+        s.w.d.dyv[kk] = dv;
       } else {
         const int kk = k - S::NDX - S::NDY;
         const int li = kk / S::WND - 2, lj = kk % S::WND - 2;
@@ -668,7 +684,6 @@ template <class T, int TW, int TH> struct Stage {
   // the update phase's bed-slope slopes (+0.5 %; in the face phase -1 %);
   // the dual kinds nowhere (Dual's final stage then spills, +65 % time).
   static constexpr bool kSelectSlopes = std::is_same<T, float>::value;
-  static constexpr bool kSelectEta = std::is_same<T, float>::value || std::is_same<T, double>::value;
   static HF_HD T pick_slope(bool zero, bool first, bool last, T l, T r) {
     const T mm = minmod(l, r);
     return zero ? cst<T>(0.0) : first ? r : last ? l : mm;
@@ -684,39 +699,29 @@ template <class T, int TW, int TH> struct Stage {
       const int l = S::idx_(li, lj), r = S::idx_(li, lj + 1);
       const bool zero = (a.nc < 2) | ((s.nd[S::i3(li, lj)] & 1) != 0);
       const bool first = gj == 0, last = gj + 1 == a.nc;
-      *se = pick_slope(zero, first, last, s.dxe[l], s.dxe[r]);
-      *su = pick_slope(zero, first, last, s.dxu[l], s.dxu[r]);
-      *sv = pick_slope(zero, first, last, s.dxv[l], s.dxv[r]);
+      *se = pick_slope(zero, first, last, s.w.d.dxe[l], s.w.d.dxe[r]);
+      *su = pick_slope(zero, first, last, s.w.d.dxu[l], s.w.d.dxu[r]);
+      *sv = pick_slope(zero, first, last, s.w.d.dxv[l], s.w.d.dxv[r]);
     } else if (a.nc < 2 || (s.nd[S::i3(li, lj)] & 1)) {
       *se = Z;
       *su = Z;
       *sv = Z;
     } else if (gj == 0) {
       const int r = S::idx_(li, lj + 1);
-      *se = s.dxe[r];
-      *su = s.dxu[r];
-      *sv = s.dxv[r];
+      *se = s.w.d.dxe[r];
+      *su = s.w.d.dxu[r];
+      *sv = s.w.d.dxv[r];
     } else if (gj + 1 == a.nc) {
       const int l = S::idx_(li, lj);
-      *se = s.dxe[l];
-      *su = s.dxu[l];
-      *sv = s.dxv[l];
+      *se = s.w.d.dxe[l];
+      *su = s.w.d.dxu[l];
+      *sv = s.w.d.dxv[l];
     } else {
       const int l = S::idx_(li, lj), r = S::idx_(li, lj + 1);
-      *se = minmod(s.dxe[l], s.dxe[r]);
-      *su = minmod(s.dxu[l], s.dxu[r]);
-      *sv = minmod(s.dxv[l], s.dxv[r]);
+      *se = minmod(s.w.d.dxe[l], s.w.d.dxe[r]);
+      *su = minmod(s.w.d.dxu[l], s.w.d.dxu[r]);
+      *sv = minmod(s.w.d.dxv[l], s.w.d.dxv[r]);
     }
-  }
-  static HF_HD T slope_x_eta(const S& s, const StageArgs<T>& a, int li, int lj, int64_t gj) {
-    if constexpr (kSelectEta) {
-      const bool zero = (a.nc < 2) | ((s.nd[S::i3(li, lj)] & 1) != 0);
-      return pick_slope(zero, gj == 0, gj + 1 == a.nc, s.dxe[S::idx_(li, lj)], s.dxe[S::idx_(li, lj + 1)]);
-    }
-    if (a.nc < 2 || (s.nd[S::i3(li, lj)] & 1)) return cst<T>(0.0);
-    if (gj == 0) return s.dxe[S::idx_(li, lj + 1)];
-    if (gj + 1 == a.nc) return s.dxe[S::idx_(li, lj)];
-    return minmod(s.dxe[S::idx_(li, lj)], s.dxe[S::idx_(li, lj + 1)]);
   }
   // update.rs:503-540
   static HF_HD void slope_y(const S& s, const StageArgs<T>& a, int li, int lj, int64_t gi, T* se, T* su,
@@ -726,39 +731,53 @@ template <class T, int TW, int TH> struct Stage {
       const int t = S::idy_(li, lj), b = S::idy_(li + 1, lj);
       const bool zero = (a.nr < 2) | ((s.nd[S::i3(li, lj)] & 2) != 0);
       const bool first = gi == 0, last = gi + 1 == a.nr;
-      *se = pick_slope(zero, first, last, s.dye[t], s.dye[b]);
-      *su = pick_slope(zero, first, last, s.dyu[t], s.dyu[b]);
-      *sv = pick_slope(zero, first, last, s.dyv[t], s.dyv[b]);
+      *se = pick_slope(zero, first, last, s.w.d.dye[t], s.w.d.dye[b]);
+      *su = pick_slope(zero, first, last, s.w.d.dyu[t], s.w.d.dyu[b]);
+      *sv = pick_slope(zero, first, last, s.w.d.dyv[t], s.w.d.dyv[b]);
     } else if (a.nr < 2 || (s.nd[S::i3(li, lj)] & 2)) {
       *se = Z;
       *su = Z;
       *sv = Z;
     } else if (gi == 0) {
       const int b = S::idy_(li + 1, lj);
-      *se = s.dye[b];
-      *su = s.dyu[b];
-      *sv = s.dyv[b];
+      *se = s.w.d.dye[b];
+      *su = s.w.d.dyu[b];
+      *sv = s.w.d.dyv[b];
     } else if (gi + 1 == a.nr) {
       const int t = S::idy_(li, lj);
-      *se = s.dye[t];
-      *su = s.dyu[t];
-      *sv = s.dyv[t];
+      *se = s.w.d.dye[t];
+      *su = s.w.d.dyu[t];
+      *sv = s.w.d.dyv[t];
     } else {
       const int t = S::idy_(li, lj), b = S::idy_(li + 1, lj);
-      *se = minmod(s.dye[t], s.dye[b]);
-      *su = minmod(s.dyu[t], s.dyu[b]);
-      *sv = minmod(s.dyv[t], s.dyv[b]);
+      *se = minmod(s.w.d.dye[t], s.w.d.dye[b]);
+      *su = minmod(s.w.d.dyu[t], s.w.d.dyu[b]);
+      *sv = minmod(s.w.d.dyv[t], s.w.d.dyv[b]);
     }
   }
-  static HF_HD T slope_y_eta(const S& s, const StageArgs<T>& a, int li, int lj, int64_t gi) {
-    if constexpr (kSelectEta) {
-      const bool zero = (a.nr < 2) | ((s.nd[S::i3(li, lj)] & 2) != 0);
-      return pick_slope(zero, gi == 0, gi + 1 == a.nr, s.dye[S::idy_(li, lj)], s.dye[S::idy_(li + 1, lj)]);
+  // Phase 3: the minmod slopes of every cell whose faces the tile needs,
+  // each formed once from the two face differences that bracket it (a face
+  // reads the slopes of its two cells, and each cell has two faces per
+  // axis), into the slope arrays; the update phase reads the eta slopes of
+  // the tile's cells from them too.
+  static HF_HD void phase_slopes(S& s, const StageArgs<T>& a, int64_t gi0, int64_t gj0, int tid, int nt) {
+    for (int k = tid; k < S::NSX + S::NSY; k += nt) {
+      T e, u, v;
+      if (k < S::NSX) {
+        const int li = k / S::WSX - 1, lj = k % S::WSX - 2;
+        slope_x(s, a, li, lj, gj0 + lj, &e, &u, &v);
+        s.sxe[k] = e;
+        s.sxu[k] = u;
+        s.sxv[k] = v;
+      } else {
+        const int kk = k - S::NSX;
+        const int li = kk / S::WSY - 2, lj = kk % S::WSY - 1;
+        slope_y(s, a, li, lj, gi0 + li, &e, &u, &v);
+        s.sye[kk] = e;
+        s.syu[kk] = u;
+        s.syv[kk] = v;
+      }
     }
-    if (a.nr < 2 || (s.nd[S::i3(li, lj)] & 2)) return cst<T>(0.0);
-    if (gi == 0) return s.dye[S::idy_(li + 1, lj)];
-    if (gi + 1 == a.nr) return s.dye[S::idy_(li, lj)];
-    return minmod(s.dye[S::idy_(li, lj)], s.dye[S::idy_(li + 1, lj)]);
   }
 
   // Raw state of a domain cell (boundary faces use it unreconstructed,
@@ -800,9 +819,9 @@ template <class T, int TW, int TH> struct Stage {
       } else if ((s.dry[S::i4(li, fj - 1)] & s.dry[S::i4(li, fj)]) == 0) {  // not both dry
         const R half_dx = R(0.5 * a.dx);
         const int l = S::i3(li, fj - 1), r = S::i3(li, fj);
-        T sle, slu, slv, sre, sru, srv;
-        slope_x(s, a, li, fj - 1, gjf - 1, &sle, &slu, &slv);
-        slope_x(s, a, li, fj, gjf, &sre, &sru, &srv);
+        const int cl = S::isx(li, fj - 1), cr = S::isx(li, fj);
+        const T sle = s.sxe[cl], slu = s.sxu[cl], slv = s.sxv[cl];
+        const T sre = s.sxe[cr], sru = s.sxu[cr], srv = s.sxv[cr];
         T u_m = s.u[l] + slu * half_dx;
         T v_m = s.v[l] + slv * half_dx;
         T u_p = s.u[r] - sru * half_dx;
@@ -823,10 +842,10 @@ template <class T, int TW, int TH> struct Stage {
         well_balanced_face(h_m, h_m * u_m, h_m * v_m, h_p, h_p * u_p, h_p * v_p, zf, &m, &na, &nb, &tg);
       }
     }
-    s.fxm[k] = m;
-    s.fxa[k] = na;
-    s.fxb[k] = nb;
-    s.fxt[k] = tg;
+    s.w.f.fxm[k] = m;
+    s.w.f.fxa[k] = na;
+    s.w.f.fxb[k] = nb;
+    s.w.f.fxt[k] = tg;
   }
 
   // One y-face (face row fi, local column lj) (update.rs:595-623, 907-922).
@@ -858,9 +877,9 @@ template <class T, int TW, int TH> struct Stage {
       } else if ((s.dry[S::i4(fi - 1, lj)] & s.dry[S::i4(fi, lj)]) == 0) {  // not both dry
         const R half_dy = R(0.5 * a.dy);
         const int t = S::i3(fi - 1, lj), b = S::i3(fi, lj);
-        T ste, stu, stv, sbe, sbu, sbv;
-        slope_y(s, a, fi - 1, lj, gif - 1, &ste, &stu, &stv);
-        slope_y(s, a, fi, lj, gif, &sbe, &sbu, &sbv);
+        const int ct = S::isy(fi - 1, lj), cb = S::isy(fi, lj);
+        const T ste = s.sye[ct], stu = s.syu[ct], stv = s.syv[ct];
+        const T sbe = s.sye[cb], sbu = s.syu[cb], sbv = s.syv[cb];
         T u_m = s.u[t] + stu * half_dy;
         T v_m = s.v[t] + stv * half_dy;
         T u_p = s.u[b] - sbu * half_dy;
@@ -881,31 +900,111 @@ template <class T, int TW, int TH> struct Stage {
         well_balanced_face(h_m, h_m * v_m, h_m * u_m, h_p, h_p * v_p, h_p * u_p, zf, &m, &na, &nb, &tg);
       }
     }
-    s.fym[k] = m;
-    s.fya[k] = na;
-    s.fyb[k] = nb;
-    s.fyt[k] = tg;
+    s.w.f.fym[k] = m;
+    s.w.f.fya[k] = na;
+    s.w.f.fyb[k] = nb;
+    s.w.f.fyt[k] = tg;
   }
 
-  // Phase 3: every x- and y-face of the tile and its 1-cell ring, in one
-  // loop so that lanes stay busy across the two face sets. Except on the
-  // 32x24 FP64 tile, the y-faces start at a multiple of 64 so that no
-  // wavefront runs both face kinds; a straddling wave costs two face
-  // evaluations and, in a loop of one to three rounds, sets the phase's
-  // latency (measured on MI355X: FP32 32x16 +3.6 %, Dual 16x8 +6 %,
-  // DualN<4> 8x8 +25 %, FP64 16x8 and 32x16 +2-4 %; FP64 32x24 -0.3 %).
-  static constexpr bool kPadFaces = !(std::is_same<T, double>::value && TW * TH >= 768);
-  static HF_HD void phase_faces(S& s, const StageArgs<T>& a, int64_t gi0, int64_t gj0, int tid, int nt) {
-    constexpr int PX = kPadFaces ? (S::NFX + 63) / 64 * 64 : S::NFX;
-    for (int k = tid; k < PX + S::NFY; k += nt) {
-      if (k < S::NFX)
-        face_x(s, a, gi0, gj0, k);
-      else if (k >= PX)
-        face_y(s, a, gi0, gj0, k - PX);
+  // One interior face (both cells in the domain) of either axis, into the
+  // face arrays (update.rs:560-623, 890-922). Faces are numbered k in
+  // [0, NFX) for x and [NFX, NFX + NFY) for y; both axes run this one code
+  // path, the axis entering only through which LDS arrays it reads (normal
+  // and tangential velocity and slopes) and the stride between its two
+  // cells, so no wavefront runs two face kinds. Faces outside the domain
+  // get zero fluxes; domain-edge faces are left to boundary_faces.
+  static HF_HD void face_interior(S& s, const StageArgs<T>& a, int64_t gi0, int64_t gj0, int k) {
+    const T Z = cst<T>(0.0);
+    const bool ya = k >= S::NFX;
+    const int kx = k, ky = ya ? k - S::NFX : 0;
+    const int rx = kx / S::WFX, ry = ky / S::WFY;
+    // local cell right of an x-face / below a y-face
+    const int li = ya ? ry - 1 : rx - 1;
+    const int lj = ya ? ky - ry * S::WFY - 1 : kx - rx * S::WFX - 1;
+    const int64_t gi = gi0 + li, gj = gj0 + lj;
+    const int64_t g_norm = ya ? gi : gj, n_norm = ya ? a.nr : a.nc;  // along the face normal
+    const int64_t g_tan = ya ? gj : gi, n_tan = ya ? a.nc : a.nr;
+    const bool inside = g_tan >= 0 && g_tan < n_tan && g_norm >= 0 && g_norm <= n_norm;
+    const bool edge = g_norm == 0 || g_norm == n_norm;
+    if (inside && edge) return;  // boundary_faces
+    T m = Z, na = Z, nb = Z, tg = Z;
+    if (inside) {
+      const int r3 = S::i3(li, lj), l3 = r3 - (ya ? S::W3 : 1);
+      const int r4 = S::i4(li, lj), l4 = r4 - (ya ? S::W4 : 1);
+      const unsigned char dl = s.dry[l4], dr = s.dry[r4];
+      T zf = interior_z_face(dl != 0, s.z[l3], dr != 0, s.z[r3]);
+      if ((dl & dr) == 0) {  // not both dry
+        const int cr = ya ? S::isy(li, lj) : S::isx(li, lj), cl = cr - (ya ? S::WSY : 1);
+        const T* vn = ya ? s.v : s.u;  // velocity normal / tangential to the face
+        const T* vt = ya ? s.u : s.v;
+        const T* se = ya ? s.sye : s.sxe;
+        const T* sn = ya ? s.syv : s.sxu;
+        const T* st = ya ? s.syu : s.sxv;
+        const R half = ya ? R(0.5 * a.dy) : R(0.5 * a.dx);
+        T n_m = vn[l3] + sn[cl] * half;
+        T t_m = vt[l3] + st[cl] * half;
+        T n_p = vn[r3] - sn[cr] * half;
+        T t_p = vt[r3] - st[cr] * half;
+        T h_m, h_p;
+        if constexpr (kRel) {
+          T dzl, dzr;
+          interior_dz(dl != 0, dr != 0, s.z[l3] - s.z[r3], &dzl, &dzr);
+          h_m = rmax(s.h[l3] + se[cl] * half + dzl, Z);
+          h_p = rmax(s.h[r3] - se[cr] * half + dzr, Z);
+          zf = Z;
+        } else {
+          T eta_m = s.h[l3] + s.z[l3] + se[cl] * half;
+          T eta_p = s.h[r3] + s.z[r3] - se[cr] * half;
+          h_m = rmax(eta_m - zf, Z);
+          h_p = rmax(eta_p - zf, Z);
+        }
+        well_balanced_face(h_m, h_m * n_m, h_m * t_m, h_p, h_p * n_p, h_p * t_p, zf, &m, &na, &nb, &tg);
+      }
+    }
+    const int kk = ya ? ky : kx;
+    T* om = ya ? s.w.f.fym : s.w.f.fxm;
+    T* oa = ya ? s.w.f.fya : s.w.f.fxa;
+    T* ob = ya ? s.w.f.fyb : s.w.f.fxb;
+    T* ot = ya ? s.w.f.fyt : s.w.f.fxt;
+    om[kk] = m;
+    oa[kk] = na;
+    ob[kk] = nb;
+    ot[kk] = tg;
+  }
+
+  // The domain-edge faces in the tile's face region (ghost cells,
+  // boundary.rs:193-367): x-faces in the west (column 0) and east (column
+  // nc) edge columns, y-faces in the north and south edge rows.
+  static HF_HD void boundary_faces(S& s, const StageArgs<T>& a, int64_t gi0, int64_t gj0, int tid, int nt) {
+    constexpr int NXB = 2 * (TH + 2), NYB = 2 * (TW + 2);
+    for (int t = tid; t < NXB + NYB; t += nt) {
+      if (t < NXB) {
+        const int li = t % (TH + 2) - 1;
+        const int64_t fj = (t < TH + 2) ? -gj0 : a.nc - gj0;
+        const int64_t gi = gi0 + li;
+        if (fj < -1 || fj > TW + 1 || gi < 0 || gi >= a.nr) continue;
+        face_x(s, a, gi0, gj0, (li + 1) * S::WFX + (int)fj + 1);
+      } else {
+        const int u = t - NXB;
+        const int lj = u % (TW + 2) - 1;
+        const int64_t fi = (u < TW + 2) ? -gi0 : a.nr - gi0;
+        const int64_t gj = gj0 + lj;
+        if (fi < -1 || fi > TH + 1 || gj < 0 || gj >= a.nc) continue;
+        face_y(s, a, gi0, gj0, ((int)fi + 1) * S::WFY + lj + 1);
+      }
     }
   }
 
-  // Phase 4: Liang-Marche outflow limiter alpha on the tile and its ring
+  // Phase 4: every x- and y-face of the tile and its 1-cell ring: interior
+  // faces in one loop over both axes, then, in tiles that reach the domain
+  // edge, the edge faces.
+  static HF_HD void phase_faces(S& s, const StageArgs<T>& a, int64_t gi0, int64_t gj0, int tid, int nt) {
+    for (int k = tid; k < S::NFX + S::NFY; k += nt) face_interior(s, a, gi0, gj0, k);
+    const bool near_edge = gi0 <= 1 || gj0 <= 1 || gi0 + TH + 1 >= a.nr || gj0 + TW + 1 >= a.nc;
+    if (near_edge) boundary_faces(s, a, gi0, gj0, tid, nt);
+  }
+
+  // Phase 5: Liang-Marche outflow limiter alpha on the tile and its ring
   // (update.rs:953-982).
   static HF_HD void phase_alpha(S& s, const StageArgs<T>& a, int64_t gi0, int64_t gj0, int tid, int nt,
                                 double dt) {
@@ -916,10 +1015,10 @@ template <class T, int TW, int TH> struct Stage {
       const int64_t gi = gi0 + li, gj = gj0 + lj;
       T al = cst<T>(1.0);
       if (gi >= 0 && gi < a.nr && gj >= 0 && gj < a.nc) {
-        T fx_right = s.fxm[S::ifx(li, lj + 1)];
-        T fx_left = s.fxm[S::ifx(li, lj)];
-        T fy_bottom = s.fym[S::ify(li + 1, lj)];
-        T fy_top = s.fym[S::ify(li, lj)];
+        T fx_right = s.w.f.fxm[S::ifx(li, lj + 1)];
+        T fx_left = s.w.f.fxm[S::ifx(li, lj)];
+        T fy_bottom = s.w.f.fym[S::ify(li + 1, lj)];
+        T fy_top = s.w.f.fym[S::ify(li, lj)];
         T out = Z;
         if (val(fx_right) > R(0)) out = out + fx_right * dt_dx;
         if (val(fx_left) < R(0)) out = out + (-fx_left) * dt_dx;
@@ -937,10 +1036,10 @@ template <class T, int TW, int TH> struct Stage {
   static HF_HD void scaled_fx(const S& s, const StageArgs<T>& a, int64_t gj0, int li, int fj, T* m, T* na,
                               T* nb, T* tg) {
     const int k = S::ifx(li, fj);
-    *m = s.fxm[k];
-    *na = s.fxa[k];
-    *nb = s.fxb[k];
-    *tg = s.fxt[k];
+    *m = s.w.f.fxm[k];
+    *na = s.w.f.fxa[k];
+    *nb = s.w.f.fxb[k];
+    *tg = s.w.f.fxt[k];
     const int64_t gjf = gj0 + fj;
     // both upstream candidates read with the face, not after its sign is known
     const T al_l = s.alpha[S::ia(li, fj - 1)], al_r = s.alpha[S::ia(li, fj)];
@@ -961,10 +1060,10 @@ template <class T, int TW, int TH> struct Stage {
   static HF_HD void scaled_fy(const S& s, const StageArgs<T>& a, int64_t gi0, int fi, int lj, T* m, T* na,
                               T* nb, T* tg) {
     const int k = S::ify(fi, lj);
-    *m = s.fym[k];
-    *na = s.fya[k];
-    *nb = s.fyb[k];
-    *tg = s.fyt[k];
+    *m = s.w.f.fym[k];
+    *na = s.w.f.fya[k];
+    *nb = s.w.f.fyb[k];
+    *tg = s.w.f.fyt[k];
     const int64_t gif = gi0 + fi;
     const T al_t = s.alpha[S::ia(fi - 1, lj)], al_b = s.alpha[S::ia(fi, lj)];
     if ((s.dry[S::i4(fi - 1, lj)] & s.dry[S::i4(fi, lj)]) != 0 && gif > 0 && gif < a.nr) return;
@@ -987,7 +1086,7 @@ template <class T, int TW, int TH> struct Stage {
                                     const T* zn4, const T* man, double* sx, double* sy) {
     const R hv_ = val(h);
     if (hv_ > R(H_DRY)) {
-      R c = rsqrt_(R(GRAVITY) * hv_);
+      R c = rsqrt_wet(R(GRAVITY) * hv_);
       R u = R(0), v = R(0);
       if (hv_ > R(H_VEL)) {
         u = div(val(hu), hv_);
@@ -1007,7 +1106,7 @@ template <class T, int TW, int TH> struct Stage {
                   xf ? a.dx : a.dy, *man, &gh, &ghu, &ghv);
       const R g = val(gh);
       if (g <= R(H_DRY)) continue;
-      R c = rsqrt_(R(GRAVITY) * g);
+      R c = rsqrt_wet(R(GRAVITY) * g);
       R u = div(val(ghu), g);
       R v = div(val(ghv), g);
       *sx = rmax(*sx, (double)(rabs(u) + c));
@@ -1113,7 +1212,7 @@ template <class T, int TW, int TH> struct Stage {
     }
   }
 
-  // Phase 5: FV update with the explicit bed-slope source
+  // Phase 6: FV update with the explicit bed-slope source
   // (update.rs:1071-1210), then the stage epilogue: write U^(1), or the
   // SSP-RK2 combine (update.rs:1272-1286) + forcing + CFL wave speeds;
   // *wet is set when an output cell is wet (not h <= H_DRY). pre: the
@@ -1154,8 +1253,8 @@ template <class T, int TW, int TH> struct Stage {
         T dhu = (ra - lb) * dt_dx + (bt - tt) * dt_dy;
         T dhv = (rt - lt) * dt_dx + (ba - tb) * dt_dy;
 
-        T sex = slope_x_eta(s, a, li, lj, gj);
-        T sey = slope_y_eta(s, a, li, lj, gi);
+        const T sex = s.sxe[S::isx(li, lj)];
+        const T sey = s.sye[S::isy(li, lj)];
         T s_hu, s_hv;
         if constexpr (kRel) {
           // Same source in the cell's own bed frame: e = eta - z_cell and

@@ -93,11 +93,44 @@ HF_HD dual rmin(dual a, dual b) {
 }
 
 HF_HD double rsqrt_(double a) { return __builtin_sqrt(a); }
+
+// sqrt of x >= 2^-767 (or +inf): the device's correctly rounded sqrt
+// sequence without its range scaling. The compiler's IEEE sqrt scales
+// x < 2^-767 up before v_rsq_f64, refines by these Newton steps, scales
+// back and passes 0 and +inf through; for x >= 2^-767 the scalings are by
+// 2^0 and the steps below are the same operations on the same operands, so
+// the result is the same bits (hf2d_math_probe checks it on the device).
+// Host builds use the library sqrt, which is correctly rounded too.
+HF_HD double sqrt_ge_tiny(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  const double r = __builtin_amdgcn_rsq(x);
+  double g = x * r, h = r * 0.5;
+  const double e = __builtin_fma(-h, g, 0.5);
+  g = __builtin_fma(g, e, g);
+  h = __builtin_fma(h, e, h);
+  double d = __builtin_fma(-g, g, x);
+  g = __builtin_fma(d, h, g);
+  d = __builtin_fma(-g, g, x);
+  g = __builtin_fma(d, h, g);
+  return x < __builtin_inf() ? g : x;
+#else
+  return __builtin_sqrt(x);
+#endif
+}
 HF_HD float rsqrt_(float a) { return fsqrt_(a); }
 HF_HD dual rsqrt_(dual a) {
   double s = __builtin_sqrt(a.v);
   double d = (s == 0.0) ? 0.0 : a.d / (2.0 * s);
   return {s, d};
+}
+
+// sqrt of a value known to be at least 2^-767 (a wet depth times g: h >
+// 1e-12): the same result as rsqrt_, without the small-argument scaling.
+HF_HD double rsqrt_wet(double a) { return sqrt_ge_tiny(a); }
+HF_HD float rsqrt_wet(float a) { return fsqrt_(a); }
+HF_HD dual rsqrt_wet(dual a) {
+  double s = sqrt_ge_tiny(a.v);
+  return {s, a.d / (2.0 * s)};
 }
 
 // x.powf(4.0 / 3.0) (source.rs:161) with the deterministic power routine.
@@ -189,6 +222,10 @@ template <int N> HF_HD dualn<N> rmin(const dualn<N>& a, const dualn<N>& b) {
 template <int N> HF_HD dualn<N> rsqrt_(const dualn<N>& a) {
   const double s = __builtin_sqrt(a.v);
   return dn_chain(a, s, s == 0.0 ? 0.0 : 0.5 / s);
+}
+template <int N> HF_HD dualn<N> rsqrt_wet(const dualn<N>& a) {
+  const double s = sqrt_ge_tiny(a.v);
+  return dn_chain(a, s, 0.5 / s);
 }
 // powf(4/3) and powf(3/5) with the deterministic powers: chain(v^n, n v^(n-1))
 template <int N> HF_HD dualn<N> pow43(const dualn<N>& x) {

@@ -123,6 +123,7 @@ void launch(const StageArgs<T>& a, int mode, unsigned long long* dst) {
     }
     for (int t = 0; t < NT; ++t) St::template phase_load<NT>(*s, a, gi0, gj0, t);
     for (int t = 0; t < NT; ++t) St::phase_diffs(*s, a, gi0, gj0, t, NT);
+    for (int t = 0; t < NT; ++t) St::phase_slopes(*s, a, gi0, gj0, t, NT);
     for (int t = 0; t < NT; ++t) St::phase_faces(*s, a, gi0, gj0, t, NT);
     for (int t = 0; t < NT; ++t) St::phase_alpha(*s, a, gi0, gj0, t, NT, dt);
     bool wet = false;
