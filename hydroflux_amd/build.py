@@ -2,11 +2,12 @@
 
     python -m hydroflux_amd.build
 
-compiles hydroflux_amd/csrc/hf2d.hip (single grids) and hf2d_batch.hip (batched
-scenarios) into hydroflux_amd/lib/libhf2d.so.
--ffp-contract=off keeps every a*b+c unfused (the reference's Rust arithmetic
-never contracts), which is what makes the f64 step bit-identical to the CPU
-restatement; packed FP32 instructions are disabled.
+compiles hydroflux_amd/csrc/hf2d.hip (single grids), hf2d_batch.hip (batched
+scenarios) and hf2d_fast.hip (the fast FP64 kind's stage kernels) into
+hydroflux_amd/lib/libhf2d.so. -ffp-contract=off keeps every a*b+c unfused in
+the first two (the reference's Rust arithmetic never contracts), which is what
+makes the f64 step bit-identical to the CPU restatement; the fast FP64 kind
+is built with contraction. Packed FP32 instructions are disabled.
 """
 from __future__ import annotations
 
@@ -32,13 +33,27 @@ def _run(cmd):
     subprocess.run(cmd, check=True, cwd=ROOT)
 
 
+# Translation units and their contraction flag: the bitwise kinds never fuse
+# a*b+c; the fast FP64 kind's kernels (hf2d_fast.hip) are built with FMA
+# contraction.
+SOURCES = (("hf2d.hip", "-ffp-contract=off"), ("hf2d_batch.hip", "-ffp-contract=off"),
+           ("hf2d_fast.hip", "-ffp-contract=fast"))
+
+
 def build_library() -> str:
     out_dir = os.path.join(HERE, "lib")
     os.makedirs(out_dir, exist_ok=True)
     out = os.path.join(out_dir, "libhf2d.so")
     tmp = out + ".tmp"
-    _run([HIPCC, *HIP_FLAGS, "-o", tmp, os.path.join(HERE, "csrc", "hf2d.hip"),
-          os.path.join(HERE, "csrc", "hf2d_batch.hip")])
+    objs = []
+    for src, contract in SOURCES:
+        obj = os.path.join(out_dir, src.replace(".hip", ".o"))
+        flags = [f for f in HIP_FLAGS if f not in ("-shared", "-ffp-contract=off")]
+        _run([HIPCC, *flags, contract, "-c", "-o", obj, os.path.join(HERE, "csrc", src)])
+        objs.append(obj)
+    _run([HIPCC, f"--offload-arch={ARCH}", "-fPIC", "-shared", "-o", tmp, *objs])
+    for obj in objs:
+        os.remove(obj)
     os.replace(tmp, out)
     return out
 

@@ -14,10 +14,12 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/hydroflux_amd.h"
 #include "hf2d_device.h"
+#include "hf2d_fast.h"
 
 
 // ============================================================== handle
@@ -244,6 +246,13 @@ template <class T, class Sh> int launch_shaped(hf2d_sim* s, const StageArgs<T>& 
 }
 
 template <class T> int launch_stage_kernel(hf2d_sim* s, const StageArgs<T>& a, int mode) {
+  if constexpr (std::is_same<T, double>::value) {
+    if (s->kind == HF2D_F64_FAST) {  // the fast FP64 build of the same kernels (hf2d_fast.hip)
+      const unsigned grid = (unsigned)((int64_t)a.tiles_x * a.tiles_y);
+      HIPCHK(s, hf2d_fast_stage_launch(&a, sizeof a, mode == hf::MODE_STAGE ? 0 : 1, s->shape, grid, s->stream));
+      return HF2D_OK;
+    }
+  }
   if constexpr (ShapeMid<T>::on) {
     if (s->shape == 1) return launch_shaped<T, ShapeMid<T>>(s, a, mode);
   }
@@ -729,7 +738,8 @@ extern "C" {
 int hf2d_create(int64_t n_rows, int64_t n_cols, double dx, double dy, int32_t scalar_kind, hf2d_sim** out) {
   if (!out) return HF2D_INVALID_ARGUMENT;
   *out = nullptr;
-  if (scalar_kind != HF2D_F64 && scalar_kind != HF2D_F32 && tangents_of(scalar_kind) == 0)
+  if (scalar_kind != HF2D_F64 && scalar_kind != HF2D_F32 && scalar_kind != HF2D_F64_FAST &&
+      tangents_of(scalar_kind) == 0)
     return HF2D_INVALID_ARGUMENT;
   if (!(dx > 0.0) || !(dy > 0.0) || n_rows <= 0 || n_cols <= 0) return HF2D_INVALID_CONFIG;
   hf2d_sim* s = new hf2d_sim();

@@ -1,0 +1,13 @@
+// Internal interface of the fast FP64 stage kernels (hf2d_fast.hip), which
+// are compiled in a translation unit of their own, with FMA contraction
+// and the fast division and square root of hf_scalar.h.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+
+// Launch the stage kernel of the fast FP64 kind: args points at a
+// StageArgs<double> (args_size bytes, checked against the kernels' own
+// layout), mode 0 = first stage, 1 = final SSP-RK2 stage, shape 0/1/2 = the
+// default / mid / small FP64 tile, grid = workgroups. Returns a hipError_t.
+hipError_t hf2d_fast_stage_launch(const void* args, size_t args_size, int mode, int shape, unsigned grid,
+                                  hipStream_t stream);

@@ -65,7 +65,22 @@ HF_HD dual operator/(dual a, double b) { return {a.v / b, a.d / b}; }
 // Division. double and dual: IEEE division, as the reference. float (the
 // precision form): reciprocal then multiply, ~1 ulp, a fraction of the
 // correctly rounded sequence's cost.
+#if defined(HF_FAST_F64) && defined(__HIP_DEVICE_COMPILE__)
+// The fast FP64 kind (its own translation unit, built with FMA
+// contraction): 1/b from v_rcp_f64 (~2^-26) and two Newton steps (~1 ulp),
+// then a multiply. The reciprocal is a pure function of b, so divisions by
+// one depth share it (hu/h, hv/h). Not correctly rounded; b = 0 gives NaN.
+HF_HD double rcp_fast(double b) {
+  double r = __builtin_amdgcn_rcp(b);
+  double e = __builtin_fma(-b, r, 1.0);
+  r = __builtin_fma(r, e, r);
+  e = __builtin_fma(-b, r, 1.0);
+  return __builtin_fma(r, e, r);
+}
+HF_HD double div(double a, double b) { return a * rcp_fast(b); }
+#else
 HF_HD double div(double a, double b) { return a / b; }
+#endif
 HF_HD dual div(dual a, dual b) { return a / b; }
 HF_HD dual div(dual a, double b) { return a / b; }
 #if defined(__HIP_DEVICE_COMPILE__) && !defined(HF_F32_IEEE)
@@ -92,7 +107,23 @@ HF_HD dual rmin(dual a, dual b) {
   return {a.v, 0.5 * (a.d + b.d)};
 }
 
+#if defined(HF_FAST_F64) && defined(__HIP_DEVICE_COMPILE__)
+// Fast FP64 kind: sqrt from v_rsq_f64 and two Newton steps on (sqrt x,
+// 1 / (2 sqrt x)) (~1 ulp), for x > 0; 0 stays 0.
+HF_HD double sqrt_fast(double x) {
+  const double r = __builtin_amdgcn_rsq(x);
+  double g = x * r, h = r * 0.5;
+  const double e = __builtin_fma(-h, g, 0.5);
+  g = __builtin_fma(g, e, g);
+  h = __builtin_fma(h, e, h);
+  const double d = __builtin_fma(-g, g, x);
+  g = __builtin_fma(d, h, g);
+  return x > 0.0 ? g : x;
+}
+HF_HD double rsqrt_(double a) { return sqrt_fast(a); }
+#else
 HF_HD double rsqrt_(double a) { return __builtin_sqrt(a); }
+#endif
 
 // sqrt of x >= 2^-767 (or +inf): the device's correctly rounded sqrt
 // sequence without its range scaling. The compiler's IEEE sqrt scales
@@ -102,7 +133,9 @@ HF_HD double rsqrt_(double a) { return __builtin_sqrt(a); }
 // the result is the same bits (hf2d_math_probe checks it on the device).
 // Host builds use the library sqrt, which is correctly rounded too.
 HF_HD double sqrt_ge_tiny(double x) {
-#if defined(__HIP_DEVICE_COMPILE__)
+#if defined(HF_FAST_F64) && defined(__HIP_DEVICE_COMPILE__)
+  return sqrt_fast(x);
+#elif defined(__HIP_DEVICE_COMPILE__)
   const double r = __builtin_amdgcn_rsq(x);
   double g = x * r, h = r * 0.5;
   const double e = __builtin_fma(-h, g, 0.5);
@@ -134,7 +167,19 @@ HF_HD dual rsqrt_wet(dual a) {
 }
 
 // x.powf(4.0 / 3.0) (source.rs:161) with the deterministic power routine.
+#if defined(HF_FAST_F64) && defined(__HIP_DEVICE_COMPILE__)
+// Fast FP64 kind: x^(4/3) = x cbrt(x) for x > 0, cbrt from an FP32
+// exp2(log2(x) / 3) seed (~2^-22) and two Newton steps in FP64
+// z <- (2 z + x / z^2) / 3.
+HF_HD double pow43(double x) {
+  double z = (double)__builtin_amdgcn_exp2f(__builtin_amdgcn_logf((float)x) * (1.0f / 3.0f));
+  z = (2.0 * z + x * rcp_fast(z * z)) * (1.0 / 3.0);
+  z = (2.0 * z + x * rcp_fast(z * z)) * (1.0 / 3.0);
+  return x * z;
+}
+#else
 HF_HD double pow43(double x) { return hfm::pow_4_3(x); }
+#endif
 HF_HD dual pow43(dual x) {
   const double n = 4.0 / 3.0;
   return {hfm::pow_4_3(x.v), n * hfm::pow_1_3m(x.v) * x.d};

@@ -212,8 +212,8 @@ class Grid2D:
     device; ``set_state``/``get_state`` are the only host<->device copies.
     """
 
-    KINDS = {"f64": 0, "dual": 1, "f32": 2, "dual2": 3, "dual4": 4, "dual8": 5, "dual16": 6}
-    TANGENTS = {"f64": 0, "dual": 1, "f32": 0, "dual2": 2, "dual4": 4, "dual8": 8, "dual16": 16}
+    KINDS = {"f64": 0, "dual": 1, "f32": 2, "dual2": 3, "dual4": 4, "dual8": 5, "dual16": 6, "f64fast": 7}
+    TANGENTS = {"f64": 0, "dual": 1, "f32": 0, "dual2": 2, "dual4": 4, "dual8": 8, "dual16": 16, "f64fast": 0}
 
     def __init__(self, n_rows: int, n_cols: int, dx: float, dy: float, dual: bool = False, kind: str = None):
         kind = kind or ("dual" if dual else "f64")
