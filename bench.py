@@ -3,7 +3,7 @@ roofline for FP64 and the FP32 precision form, the 256-scenario batch, and
 the continuous-batching serving loop's p50/p99 latency.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--size 4096] [--no-cpu]
-                    [--quick] [--acc-steps 2000] [--serve-requests 64]
+                    [--quick] [--sections ...] [--acc-steps 20000] [--serve-requests 64]
 
 Headline (`value`): the rain-on-grid flood construction — seeded fBm DEM
 (H 0.8, 50 m relief) + 0.01 slope + 8 m carved valley, +2000 m bed offset,
@@ -17,27 +17,39 @@ source, dt from the device-side CFL reduction (no host round trip), after W
 warm-up steps that wet the domain. Inputs are resident in HBM before the
 timed region.
 
+`roofline` is the final stage kernel's: algorithmic HBM bytes over its
+HIP-event launch time against 8 TB/s, and its VALU issue (lane-instructions
+per launch from the committed SQ counters over the same launch time) against
+the vector issue peak; `bound` names the larger fraction. The streaming copy
+probe's rate is the attainable HBM ceiling beside the 8 TB/s peak.
+
 Beside it in the same JSON line (unless --quick):
+  fast_f64        the fast FP64 kind (contraction, reciprocal/Newton division
+                  and square root) on the same workload, and its drift from
+                  the bitwise kind on the 256^2 dam break;
   precision_form  the same 4096^2 workload in the FP32 precision form
                   (throughput, roofline), and its depth accuracy against the
-                  FP64 run at the same simulated time after --acc-steps FP64
-                  steps, next to the FP64 run's own sensitivity to a 1e-9 m
-                  bed perturbation (the conditioning floor of the workload);
+                  FP64 run at the same simulated times, at 100, 1000, 5000 and
+                  --acc-steps FP64 steps, next to the FP64 run's own
+                  sensitivity to a 1e-9 m bed perturbation (the conditioning
+                  floor of the workload);
   c1_1024         the 1024^2 workload (no offset) in FP64 and Dual;
   batch           256 scenarios of mixed size (sides log-uniform in
                   [128, 2048], aspect U(1, 4)), each with its own DEM, land
                   cover, rain and dt: FP32 form and FP64, autotuned tile
                   shape, against serial per-scenario FP64 launches;
-  serving         Poisson arrivals of requests drawn from that mix, 25 % with
-                  Dual tangents on one land-cover class's Manning n, at a
-                  sweep of arrival rates: p50/p99 latency-to-completion and
-                  aggregate cell-updates/s;
+  serving         continuous batching of requests drawn from that mix (FP32
+                  arena holding about a third of it) at a sweep of Poisson
+                  arrival rates up to saturation: p50/p99 latency-to-completion
+                  and aggregate cell-updates/s; and the mix with 25 % Dual
+                  requests (tangents on one land-cover class's Manning n);
   dry_domain      the compacted active-tile list on the 4096^2 workload
                   without rain (inflow only), on and off;
   uk_ea           UK EA Tests 4 and 8A on their official geometries, full
                   18000 s runs, wall time;
-  cpu_baseline    the C++ restatement of the reference step, 1 thread, on a
-                  bounded 1024^2 sample.
+  cpu_baseline    the C++ restatement of the reference step, 1 thread pinned
+                  to one core: the 256^2 dam break in full, the 1024^2 flood
+                  over 3 + 50 steps, and its Dual cost ratio.
 
 One JSON line on stdout; progress on stderr.
 """
@@ -101,10 +113,28 @@ def time_steps(g, warmup, steps):
     return (t1 - t0), s1 / max(cnt, 1), s2 / max(cnt, 1)
 
 
-def roofline(kind, cells, t_first_ms, t_final_ms, traffic, kernel):
-    ab = ALG_BYTES[kind]
+# FP64 / FP32 vector issue of MI355X: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz
+# (packed FP32 is not used: one VALU instruction per lane per clock).
+VALU_PEAK_LANE_OPS = 256 * 4 * 16 * 2.4e9
+
+
+def _counters(key):
+    """VALU wave-instructions per launch of a stage kernel from the committed
+    rocprofv3 SQ counter summary (profiles/sq_counters.json)."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "sq_counters.json")) as f:
+            return json.load(f)["kernels"].get(key)
+    except (OSError, ValueError, KeyError):
+        return None
+
+
+def roofline(kind, cells, t_first_ms, t_final_ms, traffic, kernel, counter_key=None):
+    """HBM roofline of the final stage (algorithmic bytes / launch time vs
+    8 TB/s) and its VALU issue (counted lane-instructions / launch time vs
+    the vector issue peak); `bound` names the larger of the two fractions."""
+    ab = ALG_BYTES["f32" if kind == "f32" else "f64"]
     achieved = ab["final"] * cells / (t_final_ms / 1e3) / 1e9
-    return {
+    out = {
         "bound": "hbm",
         "kernel": kernel,
         "achieved": achieved,
@@ -118,42 +148,102 @@ def roofline(kind, cells, t_first_ms, t_final_ms, traffic, kernel):
         "first_stage": {"alg_bytes_per_cell": ab["stage1"], "avg_launch_ms": t_first_ms,
                         "achieved": ab["stage1"] * cells / (t_first_ms / 1e3) / 1e9},
     }
+    c = _counters(counter_key) if counter_key else None
+    if c:
+        lane = c["valu_wave_instr_per_launch"] * 64.0
+        issue = lane / (t_final_ms / 1e3)
+        out["valu"] = {"lane_instr_per_cell": lane / cells, "achieved_lane_ops_per_s": issue,
+                       "peak_lane_ops_per_s": VALU_PEAK_LANE_OPS, "issue_frac": issue / VALU_PEAK_LANE_OPS,
+                       "source": "profiles/sq_counters.json (SQ_INSTS_VALU per launch) / the live launch time"}
+        if out["valu"]["issue_frac"] > out["frac"]:
+            out["bound"] = "valu"
+    return out
 
 
-def cpu_baseline(n=1024, warmup=3, steps=20):
-    """The CPU restatement of the reference step, single-threaded as the
-    reference runs by default, on a bounded sample of the same workload
-    construction at 1024 x 1024."""
+def cpu_baseline():
+    """The CPU restatement of the reference step (the oracle), single-threaded
+    and pinned to one host core, as the reference runs by default (its rayon
+    feature is off): the 256 x 256 dam break in full (500 steps, CFL 0.5,
+    walls, cfl_time_step each step), the 1024 x 1024 flood workload over 3
+    warm-up + 50 timed steps (solver-2d/examples/m2_perf_large_grid.rs:52-73)
+    and the same in Dual over 3 + 10 steps for the f64/Dual cost ratio
+    (m1_timing_f64_vs_dual.rs). The reference's published single-thread
+    figures (282-294 ns per cell-step, i5-13420H, papers/01_review/
+    manuscript.md:684-685; Dual/f64 2.01x, manuscript.md:699) are given beside
+    it; they come from other hardware."""
     from hydroflux_amd.scenarios import flood_scenario
     from oracle.pyoracle import OracleSim, set_pow_mode, set_threads
 
-    sc = flood_scenario(n=n, dx=5.0)
-    set_threads(1)
-    set_pow_mode(0)
-    o = OracleSim(sc.n_rows, sc.n_cols, sc.dx, sc.dx)
-    o.set_mesh(sc.bed, sc.manning)
-    o.set_state(sc.dry_state())
-    o.set_boundaries(list(sc.bc_kinds))
-    o.configure(cfl=sc.cfl, max_dt=sc.max_dt, friction=True, friction_dry_tol=1e-6, rain_rate=sc.rain_rate,
-                sources=sc.sources)
-    o.run_steps(warmup)
-    t0 = time.perf_counter()
-    o.run_steps(steps)
-    dt = time.perf_counter() - t0
+    allowed = sorted(os.sched_getaffinity(0))
+    core = allowed[0]
+    os.sched_setaffinity(0, {core})
+    try:
+        set_threads(1)
+        set_pow_mode(0)
+        # 256^2 dam break, 500 steps
+        n = 256
+        o = OracleSim(n, n, 0.1, 0.1)
+        st = np.zeros((n, n, 3))
+        st[..., 0] = np.where((np.arange(n) + 0.5) * 0.1 < 12.8, 1.0, 0.1)[None, :]
+        o.set_mesh(np.zeros((n, n)), np.zeros((n, n)))
+        o.set_state(st)
+        o.set_boundaries([1, 1, 1, 1])
+        o.configure(cfl=0.5, cfl_with_bcs=False, max_dt=1e30, friction=False)
+        t0 = time.perf_counter()
+        o.run_steps(500)
+        w0 = time.perf_counter() - t0
+        o.close()
+        # 1024^2 flood, f64 and Dual
+        sc = flood_scenario(n=1024, dx=5.0)
+        rates = {}
+        for kind, steps in (("f64", 50), ("dual", 10)):
+            o = OracleSim(sc.n_rows, sc.n_cols, sc.dx, sc.dx, dual=(kind == "dual"))
+            o.set_mesh(sc.bed, sc.manning, manning_tangent=(sc.manning if kind == "dual" else None))
+            if kind == "dual":
+                o.set_state(sc.dry_state(), np.zeros_like(sc.dry_state()))
+            else:
+                o.set_state(sc.dry_state())
+            o.set_boundaries(list(sc.bc_kinds))
+            o.configure(cfl=sc.cfl, max_dt=sc.max_dt, friction=True, friction_dry_tol=1e-6,
+                        rain_rate=sc.rain_rate, sources=sc.sources)
+            o.run_steps(3)
+            t0 = time.perf_counter()
+            o.run_steps(steps)
+            rates[kind] = (time.perf_counter() - t0) / (sc.n_cells * steps)
+            o.close()
+    finally:
+        os.sched_setaffinity(0, set(allowed))
     return {
-        "value": sc.n_cells * steps / dt,
+        "value": 1.0 / rates["f64"],
         "unit": "cell-updates/s",
         "cores": 1,
         "kind": "port",
-        "sample": f"C++ restatement of the reference step (oracle), 1 thread, flood DEM {n}x{n}, "
-                  f"{warmup} warm-up + {steps} timed SSP-RK2+friction+rain+inflow steps ({dt:.1f} s)",
-        "ns_per_cell_step": dt / (sc.n_cells * steps) * 1e9,
+        "sample": "C++ restatement of the reference step (the oracle), 1 thread pinned to one core: flood DEM "
+                  "1024x1024 (rain, 100 m3/s inflow, friction), 3 warm-up + 50 timed SSP-RK2 steps",
+        "ns_per_cell_step": rates["f64"] * 1e9,
+        "dam_break_256_500_steps": {"wall_s": w0, "ns_per_cell_step": w0 / (n * n * 500) * 1e9},
+        "dual_ns_per_cell_step": rates["dual"] * 1e9,
+        "dual_over_f64": rates["dual"] / rates["f64"],
+        "published_reference": {"ns_per_cell_step": [282, 294], "dual_over_f64": 2.01,
+                                "hardware": "Intel i5-13420H, 1 thread",
+                                "source": "papers/01_review/manuscript.md:684-685, 699"},
+        "pinned_core": core,
         "host_cpu": _cpu_model(),
-        "host_cores": os.cpu_count(),
+        "host_cores": len(allowed),
     }
 
 
 def _cpu_model():
+    """The host CPU model as lscpu names it (else /proc/cpuinfo)."""
+    import subprocess
+
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                return line.split(":", 1)[1].strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
     try:
         for line in open("/proc/cpuinfo"):
             if line.startswith("model name"):
@@ -177,7 +267,11 @@ def _traffic(key):
 
 def precision_form(sc, g64, args):
     """FP32 form on the headline workload: throughput, roofline, and depth
-    accuracy against the FP64 grid `g64` (already advanced warmup + steps)."""
+    accuracy against the FP64 grid `g64` (already advanced warmup + steps)
+    at FP64 step counts 100, 1000, 5000 and --acc-steps: the FP32 form and an
+    FP64 run on the bed perturbed by 1e-9 m (N(0, 1) * 1e-9, the workload's
+    own conditioning) are each run to the FP64 run's simulated time at every
+    checkpoint and compared with it there."""
     from hydroflux_amd.metrics import depth_metrics
 
     cells = sc.n_cells
@@ -189,33 +283,83 @@ def precision_form(sc, g64, args):
         "ms_per_step": wall / args.steps * 1e3,
         "dtype": "f32 state/flux, f64 bed datum per 64x64 block, f64 dt/time/mass",
         "roofline": roofline("f32", cells, t1, t2, _traffic("f32_final_stage_bytes_per_launch"),
-                             "stage_kernel<float, final>"),
+                             "stage_kernel<float, final>", "f32_final"),
     }
+    g32.close()
     if args.acc_steps > 0:
-        done = args.warmup + args.steps
-        log(f"[bench] accuracy: FP64 to {args.acc_steps} steps, FP32 and perturbed FP64 to the same time")
-        g64.run_steps(max(0, args.acc_steps - done))
-        t_ref, n_ref = g64.time()
-        s64 = g64.get_state()[..., 0]
-        v64 = g64.total_volume()
-        g32.run_until(t_ref)
-        m32 = depth_metrics(g32.get_state()[..., 0], s64, sc.dx, sc.dx, vol_test=g32.total_volume(), vol_ref=v64)
-        n32 = g32.time()[1]
-        g32.close()
+        done = g64.time()[1]
+        marks = sorted(set(m for m in (100, 1000, 5000, args.acc_steps) if done <= m <= args.acc_steps))
+        log(f"[bench] accuracy: FP64 to {marks} steps, FP32 and perturbed FP64 to the same times")
+        g32 = build_grid(sc, "f32")
         rng = np.random.default_rng(12345)
         gp = build_grid(sc, "f64", bed=sc.bed + 1e-9 * rng.standard_normal(sc.bed.shape))
-        gp.run_until(t_ref)
-        mp = depth_metrics(gp.get_state()[..., 0], s64, sc.dx, sc.dx, vol_test=gp.total_volume(), vol_ref=v64)
-        gp.close()
-        out["accuracy"] = {
-            "t_simulated_s": t_ref,
-            "fp64_steps": n_ref,
-            "fp32_steps": n32,
-            "fp32_vs_fp64": m32,
-            "fp64_bed_perturbed_1e-9m_vs_fp64": mp,
-        }
-    else:
+        curve = []
+        for m in marks:
+            g64.run_steps(m - g64.time()[1])
+            t_ref, n_ref = g64.time()
+            s64 = g64.get_state()[..., 0]
+            v64 = g64.total_volume()
+            log(f"[bench]   FP64 at {n_ref} steps")
+            g32.run_until(t_ref)
+            log(f"[bench]   FP32 at t = {t_ref:.0f} s")
+            gp.run_until(t_ref)
+            row = {"fp64_steps": n_ref, "t_simulated_s": t_ref, "fp32_steps": g32.time()[1],
+                   "fp32_vs_fp64": depth_metrics(g32.get_state()[..., 0], s64, sc.dx, sc.dx,
+                                                 vol_test=g32.total_volume(), vol_ref=v64),
+                   "fp64_bed_perturbed_1e-9m_vs_fp64": depth_metrics(gp.get_state()[..., 0], s64, sc.dx, sc.dx,
+                                                                     vol_test=gp.total_volume(), vol_ref=v64)}
+            curve.append(row)
+            log(f"[bench]   {n_ref} steps, t = {t_ref:.0f} s: FP32 Linf {row['fp32_vs_fp64']['depth_Linf_m']:.3g} m, "
+                f"perturbed FP64 Linf {row['fp64_bed_perturbed_1e-9m_vs_fp64']['depth_Linf_m']:.3g} m")
         g32.close()
+        gp.close()
+        out["accuracy"] = curve
+    return out
+
+
+def fast_f64(sc, args, bitwise_value):
+    """The fast FP64 kind (HF2D_F64_FAST: contraction, reciprocal/Newton
+    division and square root) on the headline workload, and its drift from
+    the bitwise kind on the 256 x 256 dam break (flat frictionless bed,
+    walls, CFL 0.5) after 10, 100 and 500 steps."""
+    from hydroflux_amd import Grid2D
+
+    log("[bench] fast FP64 kind: timing")
+    g = build_grid(sc, "f64fast")
+    wall, t1, t2 = time_steps(g, args.warmup, args.steps)
+    g.close()
+    out = {"value": sc.n_cells * args.steps / wall, "ms_per_step": wall / args.steps * 1e3,
+           "over_bitwise_fp64": sc.n_cells * args.steps / wall / bitwise_value,
+           "roofline": roofline("f64", sc.n_cells, t1, t2, None, "stage_kernel<double, final> (fast build)",
+                                "f64fast_final")}
+    n = 256
+
+    def dam(kind):
+        d = Grid2D(n, n, 0.1, 0.1, kind=kind)
+        st = np.zeros((n, n, 3))
+        st[..., 0] = np.where((np.arange(n) + 0.5) * 0.1 < 12.8, 1.0, 0.1)[None, :]
+        d.set_mesh(np.zeros((n, n)), np.zeros((n, n)))
+        d.set_state(st)
+        d.set_boundaries([1, 1, 1, 1])
+        d.configure(cfl=0.5, max_dt=1e9, friction=False)
+        return d
+
+    a, b = dam("f64"), dam("f64fast")
+    v0 = b.total_volume()
+    drift, done = [], 0
+    for m in (10, 100, 500):
+        a.run_steps(m - done)
+        b.run_steps(m - done)
+        done = m
+        sa, sb = a.get_state(), b.get_state()
+        drift.append({"steps": m, "max_abs_h_m": float(np.max(np.abs(sa - sb)[..., 0])),
+                      "max_abs_q_m2s": float(np.max(np.abs(sa - sb)[..., 1:])),
+                      "mass_rel_err": abs(b.total_volume() - v0) / v0})
+    a.close()
+    b.close()
+    out["drift_vs_bitwise_dam_break_256"] = drift
+    log(f"[bench] fast FP64: {out['value'] / 1e9:.2f} G cell-updates/s ({out['over_bitwise_fp64']:.2f}x), "
+        f"dam-break drift after 500 steps {drift[-1]['max_abs_h_m']:.2g} m")
     return out
 
 
@@ -471,35 +615,58 @@ def batch_bench(args):
     return out, mix
 
 
-def serving_bench(args, mix, f32_tile):
+def serving_bench(args, mix, f32_tile, batch_f32_value):
+    """Continuous batching over requests drawn from the 256-scenario mix,
+    each running for its own simulated duration (--serve-duration, scaled
+    down from the 600-7200 s of the serving configuration so that a sweep
+    fits the bench), admitted by HBM footprint into an FP32 arena that holds
+    about a third of the mix at once:
+
+    - FP32 requests at a sweep of Poisson arrival rates up to saturation
+      (--serve-requests each; at rate inf all 256): p50/p99 latency to
+      completion and aggregate cell-updates/s, the last against the batch
+      engine running the whole mix resident;
+    - the serving configuration's mix, 25 % of the requests with Dual
+      tangents on one land-cover class's Manning n, at rate inf."""
     from hydroflux_amd.batch import ScenarioSpec
     from hydroflux_amd.serving import Request, Server, poisson_arrivals
 
     rng = np.random.default_rng(0x5E4F)
-    idx = rng.choice(len(mix), size=min(args.serve_requests, len(mix)), replace=False)
     lo, hi = args.serve_duration
-    base = []
-    for k, i in enumerate(idx):
+    durations = rng.uniform(lo, hi, len(mix))
+
+    def spec(i, dual):
         s = mix[i]
-        dual = k % 4 == 3
-        spec = ScenarioSpec(bed=s.bed, manning=s.manning, dx=s.dx, bc_kinds=s.bc_kinds, rain_rate=s.rain_rate,
-                            sources=s.sources, t_end=float(rng.uniform(lo, hi)), max_dt=s.max_dt, cfl=s.cfl,
+        return ScenarioSpec(bed=s.bed, manning=s.manning, dx=s.dx, bc_kinds=s.bc_kinds, rain_rate=s.rain_rate,
+                            sources=s.sources, t_end=float(durations[i]), max_dt=s.max_dt, cfl=s.cfl,
                             manning_tangent=((s.land_class == 2).astype(np.float64) if dual else None))
-        base.append((spec, "dual" if dual else "f32"))
-    out = {"requests": len(base), "dual_fraction": sum(k == "dual" for _, k in base) / len(base),
-           "duration_s": [lo, hi], "rates": []}
-    for rate in args.serve_rates:
-        arr = poisson_arrivals(len(base), rate=rate, seed=7)
-        reqs = [Request(k, float(a), sp, kind) for k, ((sp, kind), a) in enumerate(zip(base, arr))]
-        srv = Server({"f32": 16 << 30, "dual": 8 << 30}, max_slots=256, chunk_steps=32, keep_results=False,
+
+    cap = int(args.serve_capacity_gb * (1 << 30))
+    out = {"duration_s": [lo, hi], "f32_arena_bytes": cap, "f32_rates": []}
+
+    def serve(reqs, label):
+        srv = Server({"f32": cap, "dual": 6 << 30}, max_slots=512, chunk_steps=args.serve_chunk, keep_results=False,
                      tile_config={"f32": f32_tile})
         rep = srv.run(reqs)
         srv.close()
         s = rep.summary()
+        log(f"[bench] serving {label}: {len(reqs)} requests, p50 {s['latency_p50_s']:.2f} s, "
+            f"p99 {s['latency_p99_s']:.2f} s, {s['cell_updates_per_s'] / 1e9:.2f} G cell-updates/s")
+        return s
+
+    subset = rng.choice(len(mix), size=min(args.serve_requests, len(mix)), replace=False)
+    for rate in args.serve_rates:
+        idx = range(len(mix)) if not np.isfinite(rate) else subset
+        arr = poisson_arrivals(len(idx), rate=rate, seed=7)
+        reqs = [Request(k, float(a), spec(i, False), "f32") for k, (i, a) in enumerate(zip(idx, arr))]
+        s = serve(reqs, f"FP32 rate {rate}")
         s["rate_req_per_s"] = None if not np.isfinite(rate) else rate
-        out["rates"].append(s)
-        log(f"[bench] serving rate {rate}: p50 {s['latency_p50_s']:.2f} s, p99 {s['latency_p99_s']:.2f} s, "
-            f"{s['cell_updates_per_s'] / 1e9:.2f} G cell-updates/s")
+        out["f32_rates"].append(s)
+    sat = out["f32_rates"][-1]
+    out["f32_saturated_over_batch_engine"] = sat["cell_updates_per_s"] / batch_f32_value
+    reqs = [Request(k, 0.0, spec(int(i), k % 4 == 3), "dual" if k % 4 == 3 else "f32")
+            for k, i in enumerate(subset)]
+    out["mixed_25pct_dual_rate_inf"] = serve(reqs, "25 % Dual, rate inf")
     return out
 
 
@@ -511,15 +678,20 @@ def main():
     ap.add_argument("--size", type=int, default=4096)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="headline FP64 line only (and cpu_baseline)")
-    ap.add_argument("--sections", default="precision,c1,dry,ukea,huasco,forward,batch,serving",
-                    help="comma list of extra sections: precision, c1, dry, ukea, huasco, forward, batch, serving")
+    ap.add_argument("--sections", default="fast,precision,c1,dry,ukea,huasco,forward,batch,serving",
+                    help="comma list of extra sections: fast, precision, c1, dry, ukea, huasco, forward, batch, "
+                         "serving")
     ap.add_argument("--dry-warmup", type=int, default=500)
     ap.add_argument("--ukea-time", type=float, default=0.0,
                     help="simulated seconds of the UK EA runs (0: the full 18000 s)")
-    ap.add_argument("--acc-steps", type=int, default=2000, help="FP64 steps for the FP32 accuracy check (0: skip)")
-    ap.add_argument("--serve-requests", type=int, default=64)
-    ap.add_argument("--serve-duration", type=float, nargs=2, default=(30.0, 120.0))
-    ap.add_argument("--serve-rates", type=float, nargs="+", default=(4.0, 16.0, float("inf")))
+    ap.add_argument("--acc-steps", type=int, default=20000,
+                    help="FP64 steps of the FP32 accuracy curve (checkpoints 100, 1000, 5000 and this; 0: skip)")
+    ap.add_argument("--serve-requests", type=int, default=64, help="requests per finite arrival rate")
+    ap.add_argument("--serve-duration", type=float, nargs=2, default=(30.0, 120.0),
+                    help="simulated seconds per request, uniform in [lo, hi]")
+    ap.add_argument("--serve-rates", type=float, nargs="+", default=(8.0, 16.0, 32.0, 64.0, 128.0, float("inf")))
+    ap.add_argument("--serve-capacity-gb", type=float, default=1.25, help="FP32 serving arena (GiB)")
+    ap.add_argument("--serve-chunk", type=int, default=32, help="steps per serving chunk")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: the step runs on one GPU; --gpus {args.gpus} is not supported "
@@ -548,9 +720,12 @@ def main():
     log(f"[bench] copy probe {copy_gbs:.0f} GB/s")
 
     rf = roofline("f64", cells, t_first, t_final, _traffic("final_stage_bytes_per_launch"),
-                  "stage_kernel<double, final> (final SSP-RK2 stage + combine + forcing + CFL reduction)")
+                  "stage_kernel<double, final> (final SSP-RK2 stage + combine + forcing + CFL reduction)",
+                  "f64_final")
     rf["step_hbm_frac"] = value * ALG_BYTES["f64"]["step"] / 1e9 / HBM_PEAK_GBS
     rf["copy_probe_GBs"] = copy_gbs
+    rf["copy_probe_frac_of_peak"] = copy_gbs / HBM_PEAK_GBS
+    rf["step_frac_of_copy_probe"] = value * ALG_BYTES["f64"]["step"] / 1e9 / copy_gbs
     out = {
         "metric": METRIC,
         "value": value,
@@ -575,6 +750,8 @@ def main():
         "roofline": rf,
     }
     sections = set() if args.quick else set(x for x in args.sections.split(",") if x)
+    if "fast" in sections:
+        out["fast_f64"] = fast_f64(sc, args, value)
     if "precision" in sections:
         out["precision_form"] = precision_form(sc, g, args)
         out["precision_form"]["over_fp64"] = out["precision_form"]["value"] / value
@@ -593,10 +770,12 @@ def main():
     if "batch" in sections or "serving" in sections:
         out["batch"], mix = batch_bench(args)
         if "serving" in sections:
-            out["serving"] = serving_bench(args, mix, out["batch"]["f32"]["tile_index"])
-            out["serving"]["serial_f64"] = out["batch"]["serial_f64"]
+            sv = serving_bench(args, mix, out["batch"]["f32"]["tile_index"], out["batch"]["f32"]["value"])
+            sv["serial_f64"] = out["batch"]["serial_f64"]
+            sv["f32_saturated_over_serial_f64"] = sv["f32_rates"][-1]["cell_updates_per_s"] / sv["serial_f64"]
+            out["serving"] = sv
     if not args.no_cpu:
-        log("[bench] CPU baseline (oracle, 1 thread, 1024^2 sample)")
+        log("[bench] CPU baseline (oracle, 1 thread pinned: dam break 256^2 x 500, flood 1024^2 f64 and Dual)")
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out), flush=True)
 

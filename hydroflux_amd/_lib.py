@@ -82,6 +82,7 @@ SIGNATURES = {
     "hf2d_batch_poll": (C.c_int, [C.c_void_p, C.c_int32, _D, _I64, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                   C.POINTER(C.c_int32)]),
     "hf2d_batch_download": (C.c_int, [C.c_void_p, C.c_int32, _D, _D]),
+    "hf2d_batch_download_done": (C.c_int, [C.c_void_p, C.c_int32, _D, _D]),
     "hf2d_batch_config_count": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32)]),
     "hf2d_batch_config_info": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                          C.POINTER(C.c_int32)]),

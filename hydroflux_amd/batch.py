@@ -136,11 +136,15 @@ class Batch:
                                         used.ctypes.data_as(i32)))
         return t, steps, status, done, used
 
-    def download(self, slot: int, shape):
+    def download(self, slot: int, shape, finished: bool = False):
+        """State (and Dual tangents) of a slot. finished=True: the slot has
+        reached its end time, and its state is copied without waiting for
+        the steps still enqueued (hf2d_batch_download_done)."""
         nr, nc = shape
         s = np.empty((nr, nc, 3))
         t = np.empty_like(s) if self.kind == "dual" else None
-        self._err(lib().hf2d_batch_download(self._h, int(slot), _p(s), _p(t)))
+        fn = lib().hf2d_batch_download_done if finished else lib().hf2d_batch_download
+        self._err(fn(self._h, int(slot), _p(s), _p(t)))
         return (s, t) if self.kind == "dual" else s
 
     def free_bytes(self):

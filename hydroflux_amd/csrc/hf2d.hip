@@ -1332,8 +1332,8 @@ int hf2d_copy_probe(int64_t n_bytes, int32_t reps, double* ms_avg) {
   hipEvent_t e0, e1;
   (void)hipEventCreate(&e0);
   (void)hipEventCreate(&e1);
-  int64_t grid = (n2 + 255) / 256;
-  if (grid > 8192) grid = 8192;
+  int64_t grid = (n2 + 256 * COPY_UNROLL - 1) / (256 * COPY_UNROLL);
+  if (grid > 65536) grid = 65536;
   hipLaunchKernelGGL(copy_kernel, dim3((unsigned)grid), dim3(256), 0, 0, a, b, n2);  // warm
   (void)hipEventRecord(e0, 0);
   for (int r = 0; r < reps; ++r) hipLaunchKernelGGL(copy_kernel, dim3((unsigned)grid), dim3(256), 0, 0, a, b, n2);

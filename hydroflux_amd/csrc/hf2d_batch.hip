@@ -708,10 +708,10 @@ int hf2d_batch_poll(hf2d_batch* b, int32_t n_slots, double* t, int64_t* steps, i
   return HF2D_OK;
 }
 
-int hf2d_batch_download(hf2d_batch* b, int32_t slot, double* state_aos, double* tangent_aos) {
-  if (!b || slot < 0 || slot >= (int)b->sc.size() || !b->sc[(size_t)slot].used || !state_aos)
-    return HF2D_INVALID_ARGUMENT;
-  BCHK(b, hipStreamSynchronize(b->stream));
+namespace {
+// Copy slot's state (and Dual tangents) into AoS host arrays. The caller
+// makes sure no enqueued launch writes the slot's state.
+int copy_out(hf2d_batch* b, int32_t slot, double* state_aos, double* tangent_aos) {
   Scen& s = b->sc[(size_t)slot];
   const size_t n = (size_t)(s.nr * s.nc);
   for (int f = 0; f < 3; ++f) {
@@ -731,6 +731,26 @@ int hf2d_batch_download(hf2d_batch* b, int32_t slot, double* state_aos, double* 
     }
   }
   return HF2D_OK;
+}
+}  // namespace
+
+int hf2d_batch_download(hf2d_batch* b, int32_t slot, double* state_aos, double* tangent_aos) {
+  if (!b || slot < 0 || slot >= (int)b->sc.size() || !b->sc[(size_t)slot].used || !state_aos)
+    return HF2D_INVALID_ARGUMENT;
+  BCHK(b, hipStreamSynchronize(b->stream));
+  return copy_out(b, slot, state_aos, tangent_aos);
+}
+
+int hf2d_batch_download_done(hf2d_batch* b, int32_t slot, double* state_aos, double* tangent_aos) {
+  if (!b || slot < 0 || slot >= (int)b->sc.size() || !b->sc[(size_t)slot].used || !state_aos)
+    return HF2D_INVALID_ARGUMENT;
+  // The slot's control block, read without waiting for the batch's stream
+  // (the copy runs on the null stream, which does not synchronize with it):
+  // once done or failed, a slot's later steps are no-ops that write nothing.
+  Ctrl c;
+  BCHK(b, hipMemcpy(&c, b->ctrls + slot, sizeof(Ctrl), hipMemcpyDeviceToHost));
+  if (!(c.done || c.status)) return bfail(b, HF2D_INVALID_ARGUMENT, "slot is still running");
+  return copy_out(b, slot, state_aos, tangent_aos);
 }
 
 int hf2d_batch_config_count(const hf2d_batch* b, int32_t* count) {

@@ -4,9 +4,10 @@ Requests arrive over time (e.g. Poisson). Each is admitted into the resident
 batch of its kind (FP32 precision form for plain requests, Dual for requests
 carrying forward-mode tangents) as soon as a slot and enough HBM are free;
 its state stays on the device until its clock reaches its end time. The GPU
-advances every resident scenario together in chunks of steps; between
-chunks the host retires finished scenarios (download, free) and admits
-waiting ones, so new work overlaps with work already on the device.
+advances every resident scenario together in chunks of steps; the next
+chunk is enqueued as soon as one ends, and the host retires finished
+scenarios (download, free) and admits waiting ones while it runs, so new
+work overlaps with work already on the device.
 
 Reported: per-request latency-to-completion (completion - arrival, wall
 clock), its p50/p99, and the aggregate cell-updates/s over the run.
@@ -61,6 +62,7 @@ class ServeReport:
             "requests": len(self.requests),
             "wall_s": self.wall_s,
             "cell_updates_per_s": self.cell_updates_per_s,
+            "cell_updates_per_s_by_kind": {k: v / self.wall_s for k, v in getattr(self, "by_kind", {}).items()},
             "latency_p50_s": self.percentile(50),
             "latency_p99_s": self.percentile(99),
             "latency_mean_s": float(np.mean(self.latencies)),
@@ -92,21 +94,31 @@ class Server:
 
     def run(self, requests: List[Request], on_done: Optional[Callable[[Request], None]] = None) -> ServeReport:
         """Serve the requests to completion. on_done(request) is called as
-        each one completes (its result fields set, before it leaves the batch)."""
+        each one completes (its result fields set, before it leaves the batch).
+
+        Per batch and round: wait for the chunk in flight; remove the slots
+        that finished a round earlier (their slots and HBM are reused only
+        once no enqueued launch refers to them); read the clocks; enqueue the
+        next chunk at once, so the device works while the host downloads the
+        results of the slots that just finished (a finished slot's steps are
+        no-ops, its state is final) and admits waiting requests into free
+        slots and memory, which join at the next chunk."""
         pending = sorted(requests, key=lambda r: r.arrival)
         waiting: Dict[str, List[Request]] = {k: [] for k in self.batches}
         resident: Dict[str, Dict[int, Request]] = {k: {} for k in self.batches}
-        cell_updates = 0.0
+        retiring: Dict[str, List[int]] = {k: [] for k in self.batches}
+        in_flight = {k: False for k in self.batches}
+        cell_updates = {k: 0.0 for k in self.batches}
         i = 0
         t0 = time.perf_counter()
         tuned = set()
-        while i < len(pending) or any(waiting[k] or resident[k] for k in self.batches):
-            now = time.perf_counter() - t0
+
+        def admit(now):
+            nonlocal i
             while i < len(pending) and pending[i].arrival <= now:
                 waiting[pending[i].kind].append(pending[i])
                 i += 1
-            # admission by HBM footprint, FIFO per kind
-            for k, q in waiting.items():
+            for k, q in waiting.items():  # by HBM footprint, FIFO per kind
                 while q:
                     r = q[0]
                     try:
@@ -116,40 +128,68 @@ class Server:
                     r.admitted = time.perf_counter() - t0
                     resident[k][slot] = r
                     q.pop(0)
-            active = [k for k in self.batches if resident[k]]
-            if not active:
+
+        while i < len(pending) or any(waiting[k] or resident[k] or retiring[k] for k in self.batches):
+            admit(time.perf_counter() - t0)
+            if not any(in_flight.values()) and not any(
+                    len(resident[k]) > len(retiring[k]) for k in self.batches):
+                for k in self.batches:  # nothing running: drop what finished
+                    for slot in retiring[k]:
+                        self.batches[k].remove(slot)
+                        del resident[k][slot]
+                    retiring[k] = []
                 if i < len(pending):  # idle until the next arrival
                     time.sleep(max(0.0, min(0.01, pending[i].arrival - (time.perf_counter() - t0))))
                 continue
-            for k in active:
-                if self.autotune_steps and k not in tuned and len(resident[k]) > 1:
-                    self.batches[k].autotune(self.autotune_steps)
-                    tuned.add(k)
-                self.batches[k].step(self.chunk)
-            for k in active:
+            for k in self.batches:  # start every idle batch that has work
                 b = self.batches[k]
-                b.synchronize()
-                t, steps, status, done, used = b.poll()
-                for slot, r in list(resident[k].items()):
-                    if done[slot]:
-                        r.completed = time.perf_counter() - t0
-                        r.steps = int(steps[slot])
-                        r.sim_time = float(t[slot])
-                        r.status = int(status[slot])
-                        if self.keep:
-                            out = b.download(slot, r.spec.shape)
-                            if k == "dual":
-                                r.result, r.tangent = out
-                            else:
-                                r.result = out
-                        cell_updates += r.spec.n_cells * r.steps
-                        if on_done is not None:
-                            on_done(r)
+                if not in_flight[k] and len(resident[k]) > len(retiring[k]):
+                    if self.autotune_steps and k not in tuned and len(resident[k]) > 1:
+                        b.autotune(self.autotune_steps)
+                        tuned.add(k)
+                    b.step(self.chunk)
+                    in_flight[k] = True
+            for k in self.batches:
+                b = self.batches[k]
+                if not in_flight[k]:  # idle since its last wait: nothing refers to its finished slots
+                    for slot in retiring[k]:
                         b.remove(slot)
                         del resident[k][slot]
+                    retiring[k] = []
+                    continue
+                b.synchronize()
+                in_flight[k] = False
+                for slot in retiring[k]:  # no enqueued launch refers to them now
+                    b.remove(slot)
+                    del resident[k][slot]
+                retiring[k] = []
+                t, steps, status, done, used = b.poll()
+                finished = [slot for slot in resident[k] if done[slot]]
+                if len(resident[k]) > len(finished):
+                    b.step(self.chunk)  # the next chunk runs during the host work below
+                    in_flight[k] = True
+                for slot in finished:
+                    r = resident[k][slot]
+                    r.completed = time.perf_counter() - t0
+                    r.steps = int(steps[slot])
+                    r.sim_time = float(t[slot])
+                    r.status = int(status[slot])
+                    if self.keep:
+                        out = b.download(slot, r.spec.shape, finished=True)
+                        if k == "dual":
+                            r.result, r.tangent = out
+                        else:
+                            r.result = out
+                    cell_updates[k] += r.spec.n_cells * r.steps
+                    if on_done is not None:
+                        on_done(r)
+                    retiring[k].append(slot)
+                admit(time.perf_counter() - t0)
         wall = time.perf_counter() - t0
         lat = np.array([r.latency for r in requests])
-        return ServeReport(requests, wall, cell_updates, lat)
+        rep = ServeReport(requests, wall, sum(cell_updates.values()), lat)
+        rep.by_kind = {k: v for k, v in cell_updates.items()}
+        return rep
 
 
 def request_footprint(req: Request) -> int:

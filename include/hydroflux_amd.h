@@ -235,7 +235,7 @@ int hf2d_total_volume(hf2d_sim* sim, double* volume);
  * host threads, two handles on one device) race, so serialise them. */
 int hf2d_total_volume_noalloc(hf2d_sim* sim, double* volume);
 
-/* Bandwidth reference: average duration of a plain device copy of n_bytes
+/* Bandwidth reference: average duration of a streaming device copy of n_bytes
  * (read n_bytes, write n_bytes) over `reps` launches. Also the calibration
  * kernel for hardware traffic counters. */
 int hf2d_copy_probe(int64_t n_bytes, int32_t reps, double* ms_avg);
@@ -283,6 +283,11 @@ int hf2d_batch_synchronize(hf2d_batch* batch);
 int hf2d_batch_poll(hf2d_batch* batch, int32_t n_slots, double* t, int64_t* steps, int32_t* status,
                     int32_t* done, int32_t* used);
 int hf2d_batch_download(hf2d_batch* batch, int32_t slot, double* state_aos, double* tangent_aos);
+/* hf2d_batch_download for a slot that has finished (reached t_end or
+ * failed) without waiting for the steps still enqueued: a finished slot's
+ * later steps are no-ops, so its state is final and can be copied while
+ * they run. HF2D_INVALID_ARGUMENT if the slot is still running. */
+int hf2d_batch_download_done(hf2d_batch* batch, int32_t slot, double* state_aos, double* tangent_aos);
 
 /* Tile-shape / threads-per-workgroup configurations and the autotuner that
  * times each on the resident mix (states are restored afterwards). */
