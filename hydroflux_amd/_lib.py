@@ -79,6 +79,7 @@ SIGNATURES = {
     "hf2d_batch_remove": (C.c_int, [C.c_void_p, C.c_int32]),
     "hf2d_batch_step": (C.c_int, [C.c_void_p, C.c_int64, C.c_int32]),
     "hf2d_batch_synchronize": (C.c_int, [C.c_void_p]),
+    "hf2d_batch_query": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32)]),
     "hf2d_batch_poll": (C.c_int, [C.c_void_p, C.c_int32, _D, _I64, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                   C.POINTER(C.c_int32)]),
     "hf2d_batch_download": (C.c_int, [C.c_void_p, C.c_int32, _D, _D]),

@@ -122,6 +122,12 @@ class Batch:
     def synchronize(self):
         self._err(lib().hf2d_batch_synchronize(self._h))
 
+    def idle(self) -> bool:
+        """True when every enqueued step has finished (does not wait)."""
+        v = C.c_int32()
+        self._err(lib().hf2d_batch_query(self._h, C.byref(v)))
+        return v.value != 0
+
     def poll(self):
         """Per slot: (t, steps, status, done, used) arrays."""
         n = self.max_slots

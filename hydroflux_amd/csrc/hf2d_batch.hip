@@ -687,6 +687,18 @@ int hf2d_batch_synchronize(hf2d_batch* b) {
   return HF2D_OK;
 }
 
+int hf2d_batch_query(hf2d_batch* b, int32_t* idle) {
+  if (!b || !idle) return HF2D_INVALID_ARGUMENT;
+  const hipError_t e = hipStreamQuery(b->stream);
+  if (e == hipErrorNotReady) {
+    *idle = 0;
+    return HF2D_OK;
+  }
+  BCHK(b, e);
+  *idle = 1;
+  return HF2D_OK;
+}
+
 // Clock and status of every slot: t, steps, status (0 ok, 3 degenerate dt,
 // 4 step budget), done (reached t_end), used.
 int hf2d_batch_poll(hf2d_batch* b, int32_t n_slots, double* t, int64_t* steps, int32_t* status, int32_t* done,

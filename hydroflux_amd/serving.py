@@ -96,13 +96,15 @@ class Server:
         """Serve the requests to completion. on_done(request) is called as
         each one completes (its result fields set, before it leaves the batch).
 
-        Per batch and round: wait for the chunk in flight; remove the slots
-        that finished a round earlier (their slots and HBM are reused only
-        once no enqueued launch refers to them); read the clocks; enqueue the
-        next chunk at once, so the device works while the host downloads the
-        results of the slots that just finished (a finished slot's steps are
-        no-ops, its state is final) and admits waiting requests into free
-        slots and memory, which join at the next chunk."""
+        Each batch (FP32, Dual) runs chunks on its own stream; the host
+        serves whichever batch has finished its chunk, without waiting on
+        the other: remove the slots that finished a round earlier (their
+        slots and HBM are reused only once no enqueued launch refers to
+        them); read the clocks; enqueue the next chunk at once, so the device
+        works while the host downloads the results of the slots that just
+        finished (a finished slot's steps are no-ops, its state is final)
+        and admits waiting requests into free slots and memory, which join
+        at the next chunk."""
         pending = sorted(requests, key=lambda r: r.arrival)
         waiting: Dict[str, List[Request]] = {k: [] for k in self.batches}
         resident: Dict[str, Dict[int, Request]] = {k: {} for k in self.batches}
@@ -149,6 +151,7 @@ class Server:
                         tuned.add(k)
                     b.step(self.chunk)
                     in_flight[k] = True
+            progressed = False
             for k in self.batches:
                 b = self.batches[k]
                 if not in_flight[k]:  # idle since its last wait: nothing refers to its finished slots
@@ -157,7 +160,9 @@ class Server:
                         del resident[k][slot]
                     retiring[k] = []
                     continue
-                b.synchronize()
+                if not b.idle():  # another batch may finish first: serve whichever does
+                    continue
+                progressed = True
                 in_flight[k] = False
                 for slot in retiring[k]:  # no enqueued launch refers to them now
                     b.remove(slot)
@@ -185,6 +190,8 @@ class Server:
                         on_done(r)
                     retiring[k].append(slot)
                 admit(time.perf_counter() - t0)
+            if not progressed and any(in_flight.values()):
+                time.sleep(2e-4)
         wall = time.perf_counter() - t0
         lat = np.array([r.latency for r in requests])
         rep = ServeReport(requests, wall, sum(cell_updates.values()), lat)

@@ -278,6 +278,9 @@ int hf2d_batch_remove(hf2d_batch* batch, int32_t slot);
  * use_graph replays a captured 16-step graph. Returns without waiting. */
 int hf2d_batch_step(hf2d_batch* batch, int64_t n_steps, int32_t use_graph);
 int hf2d_batch_synchronize(hf2d_batch* batch);
+/* Whether every step enqueued on the batch has finished (*idle = 1), without
+ * waiting: lets one host thread keep several batches busy. */
+int hf2d_batch_query(hf2d_batch* batch, int32_t* idle);
 
 /* Per-slot clock and state of slots [0, n_slots): arrays may be null. */
 int hf2d_batch_poll(hf2d_batch* batch, int32_t n_slots, double* t, int64_t* steps, int32_t* status,
