@@ -59,6 +59,9 @@ struct hf2d_sim {
   int tw = 0, th = 0;  // stage tile shape: Shape<T>, ShapeMid<T> or ShapeSmall<T> (by grid size)
   int shape = 0;       // 0 Shape, 1 ShapeMid, 2 ShapeSmall
   int tiles_x = 0, tiles_y = 0;
+  // workgroups of the persistent stage kernels (FP64 kinds, default tile):
+  // CUs x resident workgroups per CU, a multiple of the 8 XCDs
+  int64_t persist_grid = 1;
   bool list_valid = false;  // tile_wet / tile_run follow the current state
   // driver forcing series ([0] rain, [1] point sources) and gauges
   SeriesDesc series_host[2] = {};
@@ -233,8 +236,25 @@ template <int N> StageArgs<hf::dualn<4>> group_args(const StageArgs<hf::dualn<N>
   return b;
 }
 
+// Workgroups of a persistent stage launch over `tiles` tiles: all that stay
+// resident at once (s->persist_grid), fewer on a grid with fewer tiles.
+unsigned persistent_grid(const hf2d_sim* s, int64_t tiles) {
+  return (unsigned)(tiles < s->persist_grid ? tiles : s->persist_grid);
+}
+
 template <class T, class Sh> int launch_shaped(hf2d_sim* s, const StageArgs<T>& a, int mode) {
   const unsigned grid = (unsigned)((int64_t)a.tiles_x * a.tiles_y);
+  if constexpr (Persist<T>::on && std::is_same<Sh, Shape<T>>::value) {
+    const unsigned pg = persistent_grid(s, grid);
+    if (mode == hf::MODE_STAGE)
+      hipLaunchKernelGGL((stage_persistent_kernel<T, Sh::TW, Sh::TH, Sh::NT, hf::MODE_STAGE>), dim3(pg),
+                         dim3(Sh::NT), 0, s->stream, a);
+    else
+      hipLaunchKernelGGL((stage_persistent_kernel<T, Sh::TW, Sh::TH, Sh::NT, hf::MODE_FINAL_RK2>), dim3(pg),
+                         dim3(Sh::NT), 0, s->stream, a);
+    HIPCHK(s, hipGetLastError());
+    return HF2D_OK;
+  }
   if (mode == hf::MODE_STAGE)
     hipLaunchKernelGGL((stage_kernel<T, Sh::TW, Sh::TH, Sh::NT, hf::MODE_STAGE>), dim3(grid), dim3(Sh::NT), 0,
                        s->stream, a);
@@ -248,7 +268,8 @@ template <class T, class Sh> int launch_shaped(hf2d_sim* s, const StageArgs<T>& 
 template <class T> int launch_stage_kernel(hf2d_sim* s, const StageArgs<T>& a, int mode) {
   if constexpr (std::is_same<T, double>::value) {
     if (s->kind == HF2D_F64_FAST) {  // the fast FP64 build of the same kernels (hf2d_fast.hip)
-      const unsigned grid = (unsigned)((int64_t)a.tiles_x * a.tiles_y);
+      const int64_t tiles = (int64_t)a.tiles_x * a.tiles_y;
+      const unsigned grid = s->shape == 0 && Persist<double>::on ? persistent_grid(s, tiles) : (unsigned)tiles;
       HIPCHK(s, hf2d_fast_stage_launch(&a, sizeof a, mode == hf::MODE_STAGE ? 0 : 1, s->shape, grid, s->stream));
       return HF2D_OK;
     }
@@ -282,6 +303,31 @@ template <class T> int choose_tiles(hf2d_sim* s) {
     s->shape = 2;
     s->tw = ShapeSmall<T>::TW;
     s->th = ShapeSmall<T>::TH;
+  }
+  return HF2D_OK;
+}
+
+// s->persist_grid: the device's CUs times the persistent stage kernels'
+// resident workgroups per CU (the smaller of the two modes'), rounded down
+// to a multiple of 8 so that every workgroup keeps its XCD's tile band.
+template <class T> int choose_persist(hf2d_sim* s) {
+  s->persist_grid = 1;
+  if constexpr (Persist<T>::on) {
+    using Sh = Shape<T>;
+    int ncu = 0, per0 = 0, per1 = 0;
+    HIPCHK(s, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, s->device));
+    if (s->kind == HF2D_F64_FAST) {
+      HIPCHK(s, hf2d_fast_stage_occupancy(&per0));
+      per1 = per0;
+    } else {
+      HIPCHK(s, hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                    &per0, stage_persistent_kernel<T, Sh::TW, Sh::TH, Sh::NT, hf::MODE_STAGE>, Sh::NT, 0));
+      HIPCHK(s, hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                    &per1, stage_persistent_kernel<T, Sh::TW, Sh::TH, Sh::NT, hf::MODE_FINAL_RK2>, Sh::NT, 0));
+    }
+    const int64_t per = per0 < per1 ? per0 : per1;
+    const int64_t g = (int64_t)ncu * (per > 0 ? per : 1);
+    s->persist_grid = g >= 8 ? g / 8 * 8 : g;
   }
   return HF2D_OK;
 }
@@ -766,6 +812,7 @@ int hf2d_create(int64_t n_rows, int64_t n_cols, double dx, double dy, int32_t sc
     return hipMalloc(p, nb) == hipSuccess && hipMemset(*p, 0, nb) == hipSuccess;
   };
   if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) return cleanup(HF2D_HIP_ERROR);
+  if (HF_DISPATCH(s, choose_persist, s) != HF2D_OK) return cleanup(HF2D_HIP_ERROR);
   for (int b = 0; b < 2; ++b)
     for (int f = 0; f < 3; ++f) {
       if (!zalloc(&s->st[b][f], bytes)) return cleanup(HF2D_HIP_ERROR);

@@ -259,6 +259,132 @@ __global__ __launch_bounds__(NT) void stage_kernel(StageArgs<T> a) {
   }
 }
 
+// Stage kernels that run as a fixed grid of workgroups, each looping over
+// tiles w = blockIdx.x, blockIdx.x + gridDim.x, ... in tile_of_block's
+// order (a multiple of 8 workgroups keeps every workgroup on its XCD's
+// band): the FP64 tile (32 x 24, 768 threads) holds 150 KB of LDS, one
+// workgroup per CU, so the per-tile kernel leaves its CU idle through each
+// load phase and each workgroup's start. Here the next tile's global loads
+// are issued right after the current tile's are stored to LDS and overlap
+// its five compute phases, and the tid-only index arithmetic is computed
+// once per workgroup instead of once per tile. Measured at 4096 x 4096
+// (flood workload, kernel time per stage): first stage 1.27 -> 1.16 ms,
+// final stage 1.57 -> 1.43 ms, bit for bit the per-tile kernel's output.
+// Per kind: on, and the phases that recompute their thread-index
+// arithmetic per tile (bit 1 load, 2 diffs, 4 slopes, 8 faces, 16 alpha, 32
+// update) in the first and the final stage. The masks are measured ones
+// (flood workload, 4096 x 4096, kernel time per stage): bitwise FP64 final
+// stage 1.57 ms per-tile, 1.43 ms with 0x0d, 1.60 ms with 0x08, 1.80 ms with
+// all phases; fast FP64 final stage 1.32 ms per-tile, 1.33 ms with 0x09,
+// 1.55 ms with 0x21. Hoisted values that overflow the VGPR budget spill to
+// scratch inside the tile loop; too many recomputed ones cost VALU issue.
+// The FP32 tile (32 x 16, 512 threads, two to three workgroups per CU) runs
+// per tile: its persistent form measured 1.13 / 1.21 ms against 0.71 /
+// 0.81 ms.
+#ifndef HF_PERSIST
+#define HF_PERSIST 1
+#endif
+#ifndef HF_PERSIST_LAUNDER_STAGE
+#define HF_PERSIST_LAUNDER_STAGE 0
+#endif
+#ifndef HF_PERSIST_LAUNDER_FINAL
+#ifdef HF_FAST_F64
+#define HF_PERSIST_LAUNDER_FINAL 0x09
+#else
+#define HF_PERSIST_LAUNDER_FINAL 0x0d
+#endif
+#endif
+template <class T> struct Persist {
+  static constexpr bool on = false;
+  static constexpr unsigned launder_stage = 0, launder_final = 0;
+};
+template <> struct Persist<double> {
+  static constexpr bool on = HF_PERSIST != 0;
+  static constexpr unsigned launder_stage = HF_PERSIST_LAUNDER_STAGE, launder_final = HF_PERSIST_LAUNDER_FINAL;
+};
+
+template <class T, int TW, int TH, int NT, int MODE>
+__global__ __launch_bounds__(NT) void stage_persistent_kernel(StageArgs<T> a) {
+  using St = hf::Stage<T, TW, TH>;
+  __shared__ typename St::S s;
+  __shared__ unsigned tile_wet_flag;
+  double dt;
+  if (!hf::stage_prologue(a, blockIdx.x == 0 && threadIdx.x == 0, &dt)) return;
+  const unsigned n = a.tile_list ? (unsigned)*a.n_work : (unsigned)((int64_t)a.tiles_x * a.tiles_y);
+  unsigned w = blockIdx.x;
+  if (w >= n) return;
+  auto tile_at = [&](unsigned ww) {
+    return a.tile_list ? (unsigned)a.tile_list[tile_of_block(ww, n)] : tile_of_block(ww, n);
+  };
+  const int tid = threadIdx.x;
+  unsigned tile = tile_at(w);
+  int64_t gi0 = (int64_t)(tile / a.tiles_x) * TH, gj0 = (int64_t)(tile % a.tiles_x) * TW;
+  typename St::template LoadRegs<NT> L;
+  typename St::Pre pre{};
+  if constexpr (St::kPrefetch && MODE == hf::MODE_STAGE) St::template prefetch<MODE>(a, gi0, gj0, tid, &pre);
+  St::template load_issue<NT>(a, gi0, gj0, tid, &L);
+  const bool tile_flags = MODE == hf::MODE_FINAL_RK2 && a.wave_speed && a.tile_wet;
+  if (tid == 0) tile_wet_flag = 0;
+  double wsx = 0.0, wsy = 0.0;
+  for (;;) {
+    // Phases whose bit is set in kLaunder take their thread index from tl,
+    // which is opaque to the compiler: their index arithmetic is recomputed
+    // per tile instead of held in registers across the loop.
+    constexpr unsigned kLaunder = MODE == hf::MODE_FINAL_RK2 ? Persist<T>::launder_final : Persist<T>::launder_stage;
+    int tl = tid;
+    if (kLaunder) asm volatile("" : "+v"(tl));
+    const int t_load = (kLaunder & 1) ? tl : tid, t_diff = (kLaunder & 2) ? tl : tid,
+              t_sl = (kLaunder & 4) ? tl : tid, t_face = (kLaunder & 8) ? tl : tid,
+              t_al = (kLaunder & 16) ? tl : tid, t_up = (kLaunder & 32) ? tl : tid;
+    St::template load_store<NT>(s, a, gi0, gj0, t_load, L);
+    typename St::Pre cur{};
+    if constexpr (St::kPrefetch) {
+      if (MODE == hf::MODE_STAGE)
+        cur = pre;
+      else
+        St::template prefetch<MODE>(a, gi0, gj0, t_up, &cur);
+    }
+    const unsigned wn = w + gridDim.x;
+    unsigned tn = 0;
+    int64_t ni0 = 0, nj0 = 0;
+    __syncthreads();
+    if (wn < n) {
+      tn = tile_at(wn);
+      ni0 = (int64_t)(tn / a.tiles_x) * TH;
+      nj0 = (int64_t)(tn % a.tiles_x) * TW;
+      St::template load_issue<NT>(a, ni0, nj0, t_load, &L);
+      if constexpr (St::kPrefetch && MODE == hf::MODE_STAGE) St::template prefetch<MODE>(a, ni0, nj0, t_up, &pre);
+    }
+    St::phase_diffs(s, a, gi0, gj0, t_diff, NT);
+    __syncthreads();
+    St::phase_slopes(s, a, gi0, gj0, t_sl, NT);
+    __syncthreads();
+    St::phase_faces(s, a, gi0, gj0, t_face, NT);
+    __syncthreads();
+    St::phase_alpha(s, a, gi0, gj0, t_al, NT, dt);
+    __syncthreads();
+    bool wet = false;
+    St::template phase_update<MODE>(s, a, gi0, gj0, t_up, NT, dt, &wsx, &wsy, &wet, St::kPrefetch ? &cur : nullptr);
+    if (tile_flags && __any(wet ? 1 : 0) && (tid & 63) == 0) tile_wet_flag = 1;
+    __syncthreads();  // LDS is reused by the next tile; the flag is complete
+    if (tile_flags && tid == 0) {
+      a.tile_wet[tile] = tile_wet_flag ? 1 : 0;
+      tile_wet_flag = 0;
+    }
+    if (wn >= n) break;
+    w = wn;
+    tile = tn;
+    gi0 = ni0;
+    gj0 = nj0;
+  }
+  if (MODE == hf::MODE_FINAL_RK2 && a.wave_speed) {
+    Ctrl* c = a.ctrl;
+    unsigned long long* dst =
+        a.ws_part ? a.ws_part + (size_t)(blockIdx.x % WS_SLOTS) * WS_STRIDE : &c->ws[c->parity ^ 1][0];
+    block_max_to(wsx, wsy, s.red, dst);
+  }
+}
+
 // Rebuild the active-tile list for the step about to run (one workgroup of
 // 1024 threads): tiles in row-major order without the ones tile_skipped()
 // leaves out; *n_work = its length. act: per-tile scratch. A step that will

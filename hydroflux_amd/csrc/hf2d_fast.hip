@@ -28,7 +28,11 @@ template <int MODE, class Sh> hipError_t launch(const hf_fast::StageArgs<double>
 template <int MODE> hipError_t launch_mode(const hf_fast::StageArgs<double>& a, int shape, unsigned grid, hipStream_t st) {
   if (shape == 1) return launch<MODE, ShapeMid<double>>(a, grid, st);
   if (shape == 2) return launch<MODE, ShapeSmall<double>>(a, grid, st);
-  return launch<MODE, Shape<double>>(a, grid, st);
+  if (!Persist<double>::on) return launch<MODE, Shape<double>>(a, grid, st);
+  using Sh = Shape<double>;
+  hipLaunchKernelGGL((stage_persistent_kernel<double, Sh::TW, Sh::TH, Sh::NT, MODE>), dim3(grid), dim3(Sh::NT), 0, st,
+                     a);
+  return hipGetLastError();
 }
 }  // namespace
 
@@ -39,4 +43,17 @@ hipError_t hf2d_fast_stage_launch(const void* args, size_t args_size, int mode, 
   hf_fast::StageArgs<double> a;
   std::memcpy(&a, args, sizeof a);
   return mode == 0 ? launch_mode<0>(a, shape, grid, stream) : launch_mode<1>(a, shape, grid, stream);
+}
+
+hipError_t hf2d_fast_stage_occupancy(int* blocks_per_cu) {
+  using Sh = Shape<double>;
+  int p0 = 0, p1 = 0;
+  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+      &p0, stage_persistent_kernel<double, Sh::TW, Sh::TH, Sh::NT, 0>, Sh::NT, 0);
+  if (e != hipSuccess) return e;
+  e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&p1, stage_persistent_kernel<double, Sh::TW, Sh::TH, Sh::NT, 1>,
+                                                   Sh::NT, 0);
+  if (e != hipSuccess) return e;
+  *blocks_per_cu = p0 < p1 ? p0 : p1;
+  return hipSuccess;
 }

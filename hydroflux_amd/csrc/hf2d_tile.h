@@ -559,14 +559,20 @@ template <class T, int TW, int TH> struct Stage {
   // issued before any is used (momentum too, used only where h > H_VEL),
   // so the phase waits for memory once rather than once per item and
   // again for the momentum.
+  // The global values a thread loads for the load phase (NT threads: up to
+  // RN items of the 4-halo each), held in registers between load_issue and
+  // load_store so that the loads of one tile can be in flight while the
+  // previous tile is computed.
+  template <int NT> struct LoadRegs {
+    static constexpr int RN = (S::N4 + NT - 1) / NT;
+    T h[RN], z[RN], hu[RN], hv[RN];
+  };
   template <int NT>
-  static HF_HD void phase_load(S& s, const StageArgs<T>& a, int64_t gi0, int64_t gj0, int tid) {
-    constexpr int RN = (S::N4 + NT - 1) / NT;
-    const T Z = cst<T>(0.0);
+  static HF_HD void load_issue(const StageArgs<T>& a, int64_t gi0, int64_t gj0, int tid, LoadRegs<NT>* L) {
+    constexpr int RN = LoadRegs<NT>::RN;
     const int64_t ci = gi0 + TH / 2 < a.nr ? gi0 + TH / 2 : a.nr - 1;
     const int64_t cj = gj0 + TW / 2 < a.nc ? gj0 + TW / 2 : a.nc - 1;
     const double tdat = tile_datum(a, ci, cj);
-    T lh[RN], lz[RN], lhu[RN], lhv[RN];
 #pragma unroll
     for (int r = 0; r < RN; ++r) {
       const int k = tid + r * NT;
@@ -574,14 +580,19 @@ template <class T, int TW, int TH> struct Stage {
       const int64_t gi = gi0 + li, gj = gj0 + lj;
       if (k < S::N4 && gi >= 0 && gi < a.nr && gj >= 0 && gj < a.nc) {
         const int64_t idx = gi * a.nc + gj;
-        lh[r] = a.ih.ld(idx);
+        L->h[r] = a.ih.ld(idx);
         if (li >= -3 && li < TH + 3 && lj >= -3 && lj < TW + 3) {
-          lz[r] = load_bed(a, gi, gj, idx, tdat);
-          lhu[r] = a.ihu.ld(idx);
-          lhv[r] = a.ihv.ld(idx);
+          L->z[r] = load_bed(a, gi, gj, idx, tdat);
+          L->hu[r] = a.ihu.ld(idx);
+          L->hv[r] = a.ihv.ld(idx);
         }
       }
     }
+  }
+  template <int NT>
+  static HF_HD void load_store(S& s, const StageArgs<T>& a, int64_t gi0, int64_t gj0, int tid, const LoadRegs<NT>& L) {
+    constexpr int RN = LoadRegs<NT>::RN;
+    const T Z = cst<T>(0.0);
 #pragma unroll
     for (int r = 0; r < RN; ++r) {
       const int k = tid + r * NT;
@@ -591,15 +602,15 @@ template <class T, int TW, int TH> struct Stage {
       const bool inside = gi >= 0 && gi < a.nr && gj >= 0 && gj < a.nc;
       const bool in3 = li >= -3 && li < TH + 3 && lj >= -3 && lj < TW + 3;
       if (inside) {
-        const T hh = lh[r];
+        const T hh = L.h[r];
         s.dry[k] = val(hh) <= R(H_DRY);
         if (in3) {
           const int k3 = S::i3(li, lj);
           s.h[k3] = hh;
-          s.z[k3] = lz[r];
+          s.z[k3] = L.z[r];
           if (val(hh) > R(H_VEL)) {
-            s.u[k3] = div(lhu[r], hh);
-            s.v[k3] = div(lhv[r], hh);
+            s.u[k3] = div(L.hu[r], hh);
+            s.v[k3] = div(L.hv[r], hh);
           } else {
             s.u[k3] = Z;
             s.v[k3] = Z;
@@ -616,6 +627,12 @@ template <class T, int TW, int TH> struct Stage {
         }
       }
     }
+  }
+  template <int NT>
+  static HF_HD void phase_load(S& s, const StageArgs<T>& a, int64_t gi0, int64_t gj0, int tid) {
+    LoadRegs<NT> L;
+    load_issue<NT>(a, gi0, gj0, tid, &L);
+    load_store<NT>(s, a, gi0, gj0, tid, L);
   }
 
   // Phase 2: per-face differences of the primitives, (R - L)/dx and
