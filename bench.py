@@ -330,7 +330,7 @@ def fast_f64(sc, args, bitwise_value):
     g.close()
     out = {"value": sc.n_cells * args.steps / wall, "ms_per_step": wall / args.steps * 1e3,
            "over_bitwise_fp64": sc.n_cells * args.steps / wall / bitwise_value,
-           "roofline": roofline("f64", sc.n_cells, t1, t2, None, "stage_kernel<double, final> (fast build)",
+           "roofline": roofline("f64", sc.n_cells, t1, t2, None, "stage_kernel<double, final> (fast build, per tile)",
                                 "f64fast_final")}
     n = 256
 
@@ -720,7 +720,7 @@ def main():
     log(f"[bench] copy probe {copy_gbs:.0f} GB/s")
 
     rf = roofline("f64", cells, t_first, t_final, _traffic("final_stage_bytes_per_launch"),
-                  "stage_kernel<double, final> (final SSP-RK2 stage + combine + forcing + CFL reduction)",
+                  "stage_persistent_kernel<double, final> (final SSP-RK2 stage + combine + forcing + CFL reduction)",
                   "f64_final")
     rf["step_hbm_frac"] = value * ALG_BYTES["f64"]["step"] / 1e9 / HBM_PEAK_GBS
     rf["copy_probe_GBs"] = copy_gbs

@@ -268,9 +268,9 @@ template <class T, class Sh> int launch_shaped(hf2d_sim* s, const StageArgs<T>& 
 template <class T> int launch_stage_kernel(hf2d_sim* s, const StageArgs<T>& a, int mode) {
   if constexpr (std::is_same<T, double>::value) {
     if (s->kind == HF2D_F64_FAST) {  // the fast FP64 build of the same kernels (hf2d_fast.hip)
-      const int64_t tiles = (int64_t)a.tiles_x * a.tiles_y;
-      const unsigned grid = s->shape == 0 && Persist<double>::on ? persistent_grid(s, tiles) : (unsigned)tiles;
-      HIPCHK(s, hf2d_fast_stage_launch(&a, sizeof a, mode == hf::MODE_STAGE ? 0 : 1, s->shape, grid, s->stream));
+      const unsigned tiles = (unsigned)((int64_t)a.tiles_x * a.tiles_y);
+      HIPCHK(s, hf2d_fast_stage_launch(&a, sizeof a, mode == hf::MODE_STAGE ? 0 : 1, s->shape, tiles,
+                                       (unsigned)s->persist_grid, s->stream));
       return HF2D_OK;
     }
   }
@@ -316,7 +316,7 @@ template <class T> int choose_persist(hf2d_sim* s) {
     using Sh = Shape<T>;
     int ncu = 0, per0 = 0, per1 = 0;
     HIPCHK(s, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, s->device));
-    if (s->kind == HF2D_F64_FAST) {
+    if (s->kind == HF2D_F64_FAST) {  // its first stage only runs persistent
       HIPCHK(s, hf2d_fast_stage_occupancy(&per0));
       per1 = per0;
     } else {

@@ -275,9 +275,10 @@ __global__ __launch_bounds__(NT) void stage_kernel(StageArgs<T> a) {
 // update) in the first and the final stage. The masks are measured ones
 // (flood workload, 4096 x 4096, kernel time per stage): bitwise FP64 final
 // stage 1.57 ms per-tile, 1.43 ms with 0x0d, 1.60 ms with 0x08, 1.80 ms with
-// all phases; fast FP64 final stage 1.32 ms per-tile, 1.33 ms with 0x09,
-// 1.55 ms with 0x21. Hoisted values that overflow the VGPR budget spill to
-// scratch inside the tile loop; too many recomputed ones cost VALU issue.
+// all phases; the fast FP64 first stage spills 15 VGPRs with 0 and none
+// with 0x09 (its final stage runs per tile). Hoisted values that overflow
+// the VGPR budget spill to scratch inside the tile loop; too many
+// recomputed ones cost VALU issue.
 // The FP32 tile (32 x 16, 512 threads, two to three workgroups per CU) runs
 // per tile: its persistent form measured 1.13 / 1.21 ms against 0.71 /
 // 0.81 ms.
@@ -285,14 +286,14 @@ __global__ __launch_bounds__(NT) void stage_kernel(StageArgs<T> a) {
 #define HF_PERSIST 1
 #endif
 #ifndef HF_PERSIST_LAUNDER_STAGE
+#ifdef HF_FAST_F64
+#define HF_PERSIST_LAUNDER_STAGE 0x09
+#else
 #define HF_PERSIST_LAUNDER_STAGE 0
 #endif
-#ifndef HF_PERSIST_LAUNDER_FINAL
-#ifdef HF_FAST_F64
-#define HF_PERSIST_LAUNDER_FINAL 0x09
-#else
-#define HF_PERSIST_LAUNDER_FINAL 0x0d
 #endif
+#ifndef HF_PERSIST_LAUNDER_FINAL
+#define HF_PERSIST_LAUNDER_FINAL 0x0d
 #endif
 template <class T> struct Persist {
   static constexpr bool on = false;

@@ -21,39 +21,47 @@
 #include <type_traits>
 
 namespace {
+// The fast kind's first stage runs persistent; its final stage runs per
+// tile: every thread-index mask tried for a persistent final stage either
+// spills 15-36 VGPRs to scratch (+0.13 GB written per launch) or is slower
+// (flood workload 4096 x 4096: per tile 1.32 ms, persistent 1.31-1.54 ms).
+constexpr bool kPersistFinal = false;
+
 template <int MODE, class Sh> hipError_t launch(const hf_fast::StageArgs<double>& a, unsigned grid, hipStream_t st) {
   hipLaunchKernelGGL((stage_kernel<double, Sh::TW, Sh::TH, Sh::NT, MODE>), dim3(grid), dim3(Sh::NT), 0, st, a);
   return hipGetLastError();
 }
-template <int MODE> hipError_t launch_mode(const hf_fast::StageArgs<double>& a, int shape, unsigned grid, hipStream_t st) {
-  if (shape == 1) return launch<MODE, ShapeMid<double>>(a, grid, st);
-  if (shape == 2) return launch<MODE, ShapeSmall<double>>(a, grid, st);
-  if (!Persist<double>::on) return launch<MODE, Shape<double>>(a, grid, st);
-  using Sh = Shape<double>;
-  hipLaunchKernelGGL((stage_persistent_kernel<double, Sh::TW, Sh::TH, Sh::NT, MODE>), dim3(grid), dim3(Sh::NT), 0, st,
-                     a);
-  return hipGetLastError();
+template <int MODE>
+hipError_t launch_mode(const hf_fast::StageArgs<double>& a, int shape, unsigned tiles, unsigned persist_grid,
+                       hipStream_t st) {
+  if (shape == 1) return launch<MODE, ShapeMid<double>>(a, tiles, st);
+  if (shape == 2) return launch<MODE, ShapeSmall<double>>(a, tiles, st);
+  if constexpr (Persist<double>::on && (MODE == 0 || kPersistFinal)) {
+    using Sh = Shape<double>;
+    const unsigned grid = tiles < persist_grid ? tiles : persist_grid;
+    hipLaunchKernelGGL((stage_persistent_kernel<double, Sh::TW, Sh::TH, Sh::NT, MODE>), dim3(grid), dim3(Sh::NT), 0,
+                       st, a);
+    return hipGetLastError();
+  } else {
+    (void)persist_grid;
+    return launch<MODE, Shape<double>>(a, tiles, st);
+  }
 }
 }  // namespace
 
-hipError_t hf2d_fast_stage_launch(const void* args, size_t args_size, int mode, int shape, unsigned grid,
-                                  hipStream_t stream) {
+hipError_t hf2d_fast_stage_launch(const void* args, size_t args_size, int mode, int shape, unsigned tiles,
+                                  unsigned persist_grid, hipStream_t stream) {
   static_assert(std::is_trivially_copyable<hf_fast::StageArgs<double>>::value, "launch arguments are plain data");
   if (args_size != sizeof(hf_fast::StageArgs<double>)) return hipErrorInvalidValue;
   hf_fast::StageArgs<double> a;
   std::memcpy(&a, args, sizeof a);
-  return mode == 0 ? launch_mode<0>(a, shape, grid, stream) : launch_mode<1>(a, shape, grid, stream);
+  return mode == 0 ? launch_mode<0>(a, shape, tiles, persist_grid, stream)
+                   : launch_mode<1>(a, shape, tiles, persist_grid, stream);
 }
 
 hipError_t hf2d_fast_stage_occupancy(int* blocks_per_cu) {
   using Sh = Shape<double>;
-  int p0 = 0, p1 = 0;
-  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-      &p0, stage_persistent_kernel<double, Sh::TW, Sh::TH, Sh::NT, 0>, Sh::NT, 0);
-  if (e != hipSuccess) return e;
-  e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&p1, stage_persistent_kernel<double, Sh::TW, Sh::TH, Sh::NT, 1>,
-                                                   Sh::NT, 0);
-  if (e != hipSuccess) return e;
-  *blocks_per_cu = p0 < p1 ? p0 : p1;
-  return hipSuccess;
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu,
+                                                      stage_persistent_kernel<double, Sh::TW, Sh::TH, Sh::NT, 0>,
+                                                      Sh::NT, 0);
 }
