@@ -697,9 +697,9 @@ template <class T, int TW, int TH> struct Stage {
   //
   // The select form reads every candidate before the choice, so the
   // difference loads do not wait behind a branch on the near-dry flag.
-  // Measured on MI355X: FP32 takes it everywhere (+3.4 %); FP64 only for
-  // the update phase's bed-slope slopes (+0.5 %; in the face phase -1 %);
-  // the dual kinds nowhere (Dual's final stage then spills, +65 % time).
+  // Measured on MI355X: FP32 takes it (+3.4 %); FP64 measured -1 % with it
+  // and Dual's final stage spilled with it (+65 % time), so the other kinds
+  // keep the branches.
   static constexpr bool kSelectSlopes = std::is_same<T, float>::value;
   static HF_HD T pick_slope(bool zero, bool first, bool last, T l, T r) {
     const T mm = minmod(l, r);
