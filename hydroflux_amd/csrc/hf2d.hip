@@ -269,8 +269,7 @@ template <class T> int launch_stage_kernel(hf2d_sim* s, const StageArgs<T>& a, i
   if constexpr (std::is_same<T, double>::value) {
     if (s->kind == HF2D_F64_FAST) {  // the fast FP64 build of the same kernels (hf2d_fast.hip)
       const unsigned tiles = (unsigned)((int64_t)a.tiles_x * a.tiles_y);
-      HIPCHK(s, hf2d_fast_stage_launch(&a, sizeof a, mode == hf::MODE_STAGE ? 0 : 1, s->shape, tiles,
-                                       (unsigned)s->persist_grid, s->stream));
+      HIPCHK(s, hf2d_fast_stage_launch(&a, sizeof a, mode == hf::MODE_STAGE ? 0 : 1, s->shape, tiles, s->stream));
       return HF2D_OK;
     }
   }
@@ -290,7 +289,11 @@ template <class T> int choose_tiles(hf2d_sim* s) {
   s->shape = 0;
   s->tw = Shape<T>::TW;
   s->th = Shape<T>::TH;
-  if (tiles_of(s, Shape<T>::TW, Shape<T>::TH) >= SMALL_GRID_TILES) return HF2D_OK;
+  if (s->kind == HF2D_F64_FAST) {  // the fast kind's own default tile (hf2d_fast.h)
+    s->tw = HF2D_FAST_TILE_W;
+    s->th = HF2D_FAST_TILE_H;
+  }
+  if (tiles_of(s, s->tw, s->th) >= SMALL_GRID_TILES) return HF2D_OK;
   if constexpr (ShapeMid<T>::on) {
     if (tiles_of(s, ShapeMid<T>::TW, ShapeMid<T>::TH) >= MID_GRID_TILES) {
       s->shape = 1;
@@ -312,19 +315,15 @@ template <class T> int choose_tiles(hf2d_sim* s) {
 // to a multiple of 8 so that every workgroup keeps its XCD's tile band.
 template <class T> int choose_persist(hf2d_sim* s) {
   s->persist_grid = 1;
+  if (s->kind == HF2D_F64_FAST) return HF2D_OK;  // one workgroup per tile (hf2d_fast.h)
   if constexpr (Persist<T>::on) {
     using Sh = Shape<T>;
     int ncu = 0, per0 = 0, per1 = 0;
     HIPCHK(s, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, s->device));
-    if (s->kind == HF2D_F64_FAST) {  // its first stage only runs persistent
-      HIPCHK(s, hf2d_fast_stage_occupancy(&per0));
-      per1 = per0;
-    } else {
-      HIPCHK(s, hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                    &per0, stage_persistent_kernel<T, Sh::TW, Sh::TH, Sh::NT, hf::MODE_STAGE>, Sh::NT, 0));
-      HIPCHK(s, hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                    &per1, stage_persistent_kernel<T, Sh::TW, Sh::TH, Sh::NT, hf::MODE_FINAL_RK2>, Sh::NT, 0));
-    }
+    HIPCHK(s, hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                  &per0, stage_persistent_kernel<T, Sh::TW, Sh::TH, Sh::NT, hf::MODE_STAGE>, Sh::NT, 0));
+    HIPCHK(s, hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                  &per1, stage_persistent_kernel<T, Sh::TW, Sh::TH, Sh::NT, hf::MODE_FINAL_RK2>, Sh::NT, 0));
     const int64_t per = per0 < per1 ? per0 : per1;
     const int64_t g = (int64_t)ncu * (per > 0 ? per : 1);
     s->persist_grid = g >= 8 ? g / 8 * 8 : g;
@@ -787,7 +786,9 @@ int hf2d_create(int64_t n_rows, int64_t n_cols, double dx, double dy, int32_t sc
   if (scalar_kind != HF2D_F64 && scalar_kind != HF2D_F32 && scalar_kind != HF2D_F64_FAST &&
       tangents_of(scalar_kind) == 0)
     return HF2D_INVALID_ARGUMENT;
-  if (!(dx > 0.0) || !(dy > 0.0) || n_rows <= 0 || n_cols <= 0) return HF2D_INVALID_CONFIG;
+  if (!(dx > 0.0) || !(dy > 0.0) || n_rows <= 0 || n_cols <= 0 || n_rows >= HF2D_MAX_SIDE ||
+      n_cols >= HF2D_MAX_SIDE)
+    return HF2D_INVALID_CONFIG;
   hf2d_sim* s = new hf2d_sim();
   if (hipGetDevice(&s->device) != hipSuccess) {
     delete s;

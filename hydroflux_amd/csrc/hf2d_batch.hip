@@ -391,7 +391,7 @@ template <class T> int init_wave_speeds(hf2d_batch* b, int slot) {
   a.n_src = 0;
   a.wave_speed = 1;
   hipLaunchKernelGGL(zero_slot_kernel, dim3(1), dim3(1), 0, b->stream, a.ctrl);
-  int64_t g = (a.nr * a.nc + 255) / 256;
+  int64_t g = ((int64_t)a.nr * a.nc + 255) / 256;
   if (g > 65536) g = 65536;
   hipLaunchKernelGGL((tail_kernel<T>), dim3((unsigned)g), dim3(256), 0, b->stream, a, 1);
   BCHK(b, hipGetLastError());
@@ -543,7 +543,8 @@ int hf2d_batch_add(hf2d_batch* b, int64_t nr, int64_t nc, double dx, double dy, 
                    const hf2d_config* cfg, const hf2d_point_source* srcs, int64_t n_src, double t_end,
                    int32_t* slot_out) {
   if (!b || !bed || !manning || !state_aos || !bc_kinds || !cfg || !slot_out) return HF2D_INVALID_ARGUMENT;
-  if (!(dx > 0.0) || !(dy > 0.0) || nr <= 0 || nc <= 0) return bfail(b, HF2D_INVALID_CONFIG, "bad grid");
+  if (!(dx > 0.0) || !(dy > 0.0) || nr <= 0 || nc <= 0 || nr >= HF2D_MAX_SIDE || nc >= HF2D_MAX_SIDE)
+    return bfail(b, HF2D_INVALID_CONFIG, "bad grid");
   if (!(cfg->cfl > 0.0 && cfg->cfl <= 1.0) || !(cfg->max_dt > 0.0) || cfg->integrator != HF2D_SSPRK2)
     return bfail(b, HF2D_INVALID_CONFIG, "batch scenarios need cfl in (0,1], max_dt > 0 and SSP-RK2");
   for (int64_t k = 0; k < n_src; ++k)

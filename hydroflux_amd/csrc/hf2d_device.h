@@ -233,8 +233,8 @@ __global__ __launch_bounds__(NT) void stage_kernel(StageArgs<T> a) {
   } else {
     tile = tile_of_block(blockIdx.x, gridDim.x);
   }
-  const int64_t ty = tile / a.tiles_x, tx = tile % a.tiles_x;
-  const int64_t gi0 = ty * TH, gj0 = tx * TW;
+  const int ty = (int)(tile / (unsigned)a.tiles_x), tx = (int)(tile % (unsigned)a.tiles_x);
+  const int gi0 = ty * TH, gj0 = tx * TW;
   const int tid = threadIdx.x;
   typename St::Pre pre{};
   if constexpr (St::kPrefetch) St::template prefetch<MODE>(a, gi0, gj0, tid, &pre);
@@ -275,22 +275,20 @@ __global__ __launch_bounds__(NT) void stage_kernel(StageArgs<T> a) {
 // update) in the first and the final stage. The masks are measured ones
 // (flood workload, 4096 x 4096, kernel time per stage): bitwise FP64 final
 // stage 1.57 ms per-tile, 1.43 ms with 0x0d, 1.60 ms with 0x08, 1.80 ms with
-// all phases; the fast FP64 first stage spills 15 VGPRs with 0 and none
-// with 0x09 (its final stage runs per tile). Hoisted values that overflow
-// the VGPR budget spill to scratch inside the tile loop; too many
-// recomputed ones cost VALU issue.
-// The FP32 tile (32 x 16, 512 threads, two to three workgroups per CU) runs
-// per tile: its persistent form measured 1.13 / 1.21 ms against 0.71 /
-// 0.81 ms.
+// all phases. Hoisted values that overflow the VGPR budget spill to scratch
+// inside the tile loop; too many recomputed ones cost VALU issue.
+// The bitwise FP64 kernels keep this form. Tiles of half the LDS (24 x 16 or
+// 32 x 12 with 512 threads, two workgroups per CU) measured 2.62 ms per step
+// one workgroup per tile and 2.95-3.16 ms persistent (capped at 128 VGPRs),
+// against 2.57 ms here. The fast FP64 kind takes the 32 x 12 per-tile form
+// (hf2d_fast.h), and the FP32 tile (32 x 16, 512 threads, two to three
+// workgroups per CU) runs per tile: its persistent form measured 1.13 /
+// 1.21 ms against 0.71 / 0.81 ms.
 #ifndef HF_PERSIST
 #define HF_PERSIST 1
 #endif
 #ifndef HF_PERSIST_LAUNDER_STAGE
-#ifdef HF_FAST_F64
-#define HF_PERSIST_LAUNDER_STAGE 0x09
-#else
 #define HF_PERSIST_LAUNDER_STAGE 0
-#endif
 #endif
 #ifndef HF_PERSIST_LAUNDER_FINAL
 #define HF_PERSIST_LAUNDER_FINAL 0x0d
@@ -319,7 +317,7 @@ __global__ __launch_bounds__(NT) void stage_persistent_kernel(StageArgs<T> a) {
   };
   const int tid = threadIdx.x;
   unsigned tile = tile_at(w);
-  int64_t gi0 = (int64_t)(tile / a.tiles_x) * TH, gj0 = (int64_t)(tile % a.tiles_x) * TW;
+  int gi0 = (int)(tile / (unsigned)a.tiles_x) * TH, gj0 = (int)(tile % (unsigned)a.tiles_x) * TW;
   typename St::template LoadRegs<NT> L;
   typename St::Pre pre{};
   if constexpr (St::kPrefetch && MODE == hf::MODE_STAGE) St::template prefetch<MODE>(a, gi0, gj0, tid, &pre);
@@ -347,12 +345,12 @@ __global__ __launch_bounds__(NT) void stage_persistent_kernel(StageArgs<T> a) {
     }
     const unsigned wn = w + gridDim.x;
     unsigned tn = 0;
-    int64_t ni0 = 0, nj0 = 0;
+    int ni0 = 0, nj0 = 0;
     __syncthreads();
     if (wn < n) {
       tn = tile_at(wn);
-      ni0 = (int64_t)(tn / a.tiles_x) * TH;
-      nj0 = (int64_t)(tn % a.tiles_x) * TW;
+      ni0 = (int)(tn / (unsigned)a.tiles_x) * TH;
+      nj0 = (int)(tn % (unsigned)a.tiles_x) * TW;
       St::template load_issue<NT>(a, ni0, nj0, t_load, &L);
       if constexpr (St::kPrefetch && MODE == hf::MODE_STAGE) St::template prefetch<MODE>(a, ni0, nj0, t_up, &pre);
     }
@@ -469,11 +467,11 @@ __global__ __launch_bounds__(256) void tail_kernel(StageArgs<T> a, int slot_mode
   if (!hf::stage_prologue(a, blockIdx.x == 0 && threadIdx.x == 0, &dt)) return;
   using St = hf::Stage<T, 32, 16>;
   double wsx = 0.0, wsy = 0.0;
-  const int64_t n = a.nr * a.nc;
+  const int64_t n = (int64_t)a.nr * a.nc;
   const typename St::ForcingNow fnow = St::forcing_now(a, 0, 0, a.nr, a.nc);
   for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < n;
        idx += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t gi = idx / a.nc, gj = idx % a.nc;
+    const int gi = (int)(idx / a.nc), gj = (int)(idx % a.nc);
     T h = a.ih.ld(idx), hu = a.ihu.ld(idx), hv = a.ihv.ld(idx);
     St::forcing(a, fnow, gi, gj, dt, &h, &hu, &hv);
     a.oh.st(idx, h);
@@ -782,7 +780,7 @@ __global__ __launch_bounds__(NT) void batch_stage_kernel(const StageArgs<T>* __r
     const StageArgs<T>& a = descs[it.slot];
     double dt;
     if (!hf::stage_prologue(a, it.tile == 0 && tid == 0, &dt)) continue;
-    const int64_t gi0 = (int64_t)(it.tile / a.tiles_x) * TH, gj0 = (int64_t)(it.tile % a.tiles_x) * TW;
+    const int gi0 = (int)(it.tile / a.tiles_x) * TH, gj0 = (int)(it.tile % a.tiles_x) * TW;
     typename St::Pre pre{};
     if constexpr (St::kPrefetch) St::template prefetch<MODE>(a, gi0, gj0, tid, &pre);
     St::template phase_load<NT>(s, a, gi0, gj0, tid);

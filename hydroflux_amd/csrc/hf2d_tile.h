@@ -181,7 +181,10 @@ struct PointSrc {  // source.rs:36-44
 // Launch arguments of a stage: the members that do not depend on the
 // scalar type, then (StageArgs<T>) the field views.
 struct StageCommon {
-  int64_t nr, nc;
+  // Grid size. Both sides are below 2^30 (hf2d_create and the batch engine
+  // reject larger ones), so cell coordinates and tile origins are 32-bit;
+  // element offsets into the planes are formed in 64 bits.
+  int nr, nc;
   double dx, dy;
   int bc_kind[4];  // N, S, W, E
   double bc_param[4];
@@ -234,17 +237,17 @@ template <class T> struct StageArgs : StageCommon {
 // the FP64 datum of the cell's block minus the tile datum, plus the FP32
 // deviation, summed in FP64 and rounded once, so a tile on 2000 m relief
 // works with beds of a few metres and h + z keeps its FP32 digits.
-template <class T> HF_HD T load_bed(const StageArgs<T>& a, int64_t gi, int64_t gj, int64_t idx, double) {
+template <class T> HF_HD T load_bed(const StageArgs<T>& a, int gi, int gj, int64_t idx, double) {
   return a.z.ld(idx);
 }
-template <> HF_HD float load_bed<float>(const StageArgs<float>& a, int64_t gi, int64_t gj, int64_t idx,
+template <> HF_HD float load_bed<float>(const StageArgs<float>& a, int gi, int gj, int64_t idx,
                                         double tile_datum) {
-  const double d = a.datum[(gi >> a.datum_log2) * a.datum_cols + (gj >> a.datum_log2)];
+  const double d = a.datum[(int64_t)(gi >> a.datum_log2) * a.datum_cols + (gj >> a.datum_log2)];
   return (float)((d - tile_datum) + (double)a.zdev[idx]);
 }
-template <class T> HF_HD double tile_datum(const StageArgs<T>&, int64_t, int64_t) { return 0.0; }
-template <> HF_HD double tile_datum<float>(const StageArgs<float>& a, int64_t gi, int64_t gj) {
-  return a.datum[(gi >> a.datum_log2) * a.datum_cols + (gj >> a.datum_log2)];
+template <class T> HF_HD double tile_datum(const StageArgs<T>&, int, int) { return 0.0; }
+template <> HF_HD double tile_datum<float>(const StageArgs<float>& a, int gi, int gj) {
+  return a.datum[(int64_t)(gi >> a.datum_log2) * a.datum_cols + (gj >> a.datum_log2)];
 }
 
 // ------------------------------------------------------------ small pieces
@@ -404,6 +407,95 @@ HF_HD void well_balanced_face(T lh, T lqn, T lqt, T rh, T rqn, T rqt, T zf, T* m
   *tang = ft;
 }
 
+// Kinds held to accuracy bounds rather than to the reference's bits: the
+// FP32 precision form, and the fast FP64 kind (HF_FAST_F64, a translation
+// unit of its own). Their interior faces run prim_face below.
+#ifndef HF_PRIM_FACE
+#define HF_PRIM_FACE 1
+#endif
+#if defined(HF_FAST_F64)
+template <class T> struct PrimFace {
+  static constexpr bool on = HF_PRIM_FACE && (std::is_same<T, float>::value || std::is_same<T, double>::value);
+};
+#else
+template <class T> struct PrimFace {
+  static constexpr bool on = HF_PRIM_FACE && std::is_same<T, float>::value;
+};
+#endif
+
+// well_balanced_face + hllc_normal_flux (update.rs:256-353, riemann.rs:
+// 114-204) in the face's primitive states (depth h, normal and tangential
+// velocity un, ut) for the kinds of PrimFace. With z_left = z_right = z_face
+// the reference's hydrostatic depth max(h + z_face - z_face, 0) is h, its
+// velocity (h u) / h and the HLLC's un = (h un) / h are un, and the
+// correction (h^2 - hs^2) g / 2 vanishes, all in exact arithmetic; so the
+// reconstructed velocities are used as they are, with the reference's
+// cut-offs (velocity 0 at h <= H_VEL, a side dry at h <= DRY_TOL), and the
+// eight divisions and two corrections of the conserved form drop out. Both
+// normal-momentum fluxes are *nm.
+template <class T>
+HF_HD void prim_face(T h_l, T un_l, T ut_l, T h_r, T un_r, T ut_r, T* fm, T* fn, T* ft) {
+  using R = typename ScalarOf<T>::type;
+  const T Z = cst<T>(0.0);
+  const bool lw = val(h_l) > R(DRY_TOL);
+  const bool rw = val(h_r) > R(DRY_TOL);
+  if (!lw && !rw) {
+    *fm = Z;
+    *fn = Z;
+    *ft = Z;
+    return;
+  }
+  const bool lv = val(h_l) > R(H_VEL), rv = val(h_r) > R(H_VEL);
+  un_l = lv ? un_l : Z;
+  ut_l = lv ? ut_l : Z;
+  un_r = rv ? un_r : Z;
+  ut_r = rv ? ut_r : Z;
+  T cl = lw ? rsqrt_wet(h_l * R(GRAVITY)) : Z;
+  T cr = rw ? rsqrt_wet(h_r * R(GRAVITY)) : Z;
+  T sl, sr;
+  if (lw && rw) {
+    sl = rmin(un_l - cl, un_r - cr);
+    sr = rmax(un_l + cl, un_r + cr);
+  } else if (rw) {
+    sl = un_r - cr * R(2.0);
+    sr = un_r + cr;
+  } else {
+    sl = un_l - cl;
+    sr = un_l + cl * R(2.0);
+  }
+  const bool exit_l = val(sl) >= R(0);
+  const bool star = !exit_l && !(val(sr) <= R(0));
+  T s_star = Z;
+  if (star) {
+    const T wr = h_r * (un_r - sr), wl = h_l * (un_l - sl);
+    const T denom = wr - wl;
+    s_star = (rabs(val(denom)) > R(1.0e-12)) ? div(wr * sl - wl * sr, denom) : (sl + sr) * R(0.5);
+  }
+  const bool left = star ? val(s_star) >= R(0) : exit_l;
+  const bool wK = left ? lw : rw;
+  const T hK = left ? h_l : h_r;
+  const T unK = left ? un_l : un_r;
+  const T utK = left ? ut_l : ut_r;
+  const T qnK = hK * unK;
+  T fmK = Z, fnK = Z, ftK = Z;
+  if (wK) {
+    fmK = qnK;
+    fnK = qnK * unK + hK * hK * R(0.5 * GRAVITY);
+    ftK = qnK * utK;
+  }
+  if (!star) {
+    *fm = fmK;
+    *fn = fnK;
+    *ft = ftK;
+    return;
+  }
+  const T sK = left ? sl : sr;
+  const T h_star = div(hK * (sK - unK), sK - s_star);
+  *fm = fmK + (h_star - hK) * sK;
+  *fn = fnK + (h_star * s_star - qnK) * sK;
+  *ft = ftK + (h_star - hK) * utK * sK;
+}
+
 // ------------------------------------------------------------ tile LDS
 template <class T, int TW, int TH> struct TileSmem {
   static constexpr int W3 = TW + 6, H3 = TH + 6, N3 = W3 * H3;
@@ -425,7 +517,15 @@ template <class T, int TW, int TH> struct TileSmem {
   static constexpr int WSY = TW + 2, HSY = TH + 4, NSY = WSY * HSY;
 
   T h[N3], z[N3], u[N3], v[N3];
-  T sxe[NSX], sxu[NSX], sxv[NSX];
+  T sxe[NSX];
+  // The u-slopes along x are last read by the face phase; alpha (tile +
+  // ring, written by the alpha phase after it) takes their LDS. The tile
+  // then fits in half a CU's LDS at more shapes (FP64 24 x 16: 77 KB).
+  union {
+    T sxu[NSX];
+    T alpha[NA];
+  };
+  T sxv[NSX];
   T sye[NSY], syu[NSY], syv[NSY];
   // The face differences are read only to form the slopes, and the face
   // fluxes are written after that: one block of LDS holds the one, then
@@ -442,10 +542,11 @@ template <class T, int TW, int TH> struct TileSmem {
     Diffs d;
     Fluxes f;
   } w;
-  T alpha[NA];
   unsigned char dry[N4];
   unsigned char nd[N3];  // bit 0: a dry cell within +-2 along x; bit 1: along y
   double red[3][16];
+
+  static_assert(NA <= NSX, "alpha lives in the u-slope array");
 
   static HF_HD int i3(int li, int lj) {
     HF_BOUNDS_CHECK(li >= -3 && li < TH + 3 && lj >= -3 && lj < TW + 3);
@@ -520,8 +621,8 @@ template <class T, int TW, int TH> struct Stage {
   }
 
   // Face bed of x-face fj in local row li (update.rs:676-700).
-  static HF_HD T zface_x(const S& s, const StageArgs<T>& a, int64_t gi0, int64_t gj0, int li, int fj) {
-    const int64_t gjf = gj0 + fj;
+  static HF_HD T zface_x(const S& s, const StageArgs<T>& a, int gi0, int gj0, int li, int fj) {
+    const int gjf = gj0 + fj;
     if (gjf == 0) {
       T zc = s.z[S::i3(li, fj)];
       T zn = (a.nc >= 2) ? s.z[S::i3(li, fj + 1)] : zc;
@@ -536,8 +637,8 @@ template <class T, int TW, int TH> struct Stage {
                            s.dry[S::i4(li, fj)] != 0, s.z[S::i3(li, fj)]);
   }
   // Face bed of y-face fi in local column lj (update.rs:704-728).
-  static HF_HD T zface_y(const S& s, const StageArgs<T>& a, int64_t gi0, int64_t gj0, int fi, int lj) {
-    const int64_t gif = gi0 + fi;
+  static HF_HD T zface_y(const S& s, const StageArgs<T>& a, int gi0, int gj0, int fi, int lj) {
+    const int gif = gi0 + fi;
     if (gif == 0) {
       T zc = s.z[S::i3(fi, lj)];
       T zn = (a.nr >= 2) ? s.z[S::i3(fi + 1, lj)] : zc;
@@ -568,18 +669,18 @@ template <class T, int TW, int TH> struct Stage {
     T h[RN], z[RN], hu[RN], hv[RN];
   };
   template <int NT>
-  static HF_HD void load_issue(const StageArgs<T>& a, int64_t gi0, int64_t gj0, int tid, LoadRegs<NT>* L) {
+  static HF_HD void load_issue(const StageArgs<T>& a, int gi0, int gj0, int tid, LoadRegs<NT>* L) {
     constexpr int RN = LoadRegs<NT>::RN;
-    const int64_t ci = gi0 + TH / 2 < a.nr ? gi0 + TH / 2 : a.nr - 1;
-    const int64_t cj = gj0 + TW / 2 < a.nc ? gj0 + TW / 2 : a.nc - 1;
+    const int ci = gi0 + TH / 2 < a.nr ? gi0 + TH / 2 : a.nr - 1;
+    const int cj = gj0 + TW / 2 < a.nc ? gj0 + TW / 2 : a.nc - 1;
     const double tdat = tile_datum(a, ci, cj);
 #pragma unroll
     for (int r = 0; r < RN; ++r) {
       const int k = tid + r * NT;
       const int li = k / S::W4 - 4, lj = k % S::W4 - 4;
-      const int64_t gi = gi0 + li, gj = gj0 + lj;
+      const int gi = gi0 + li, gj = gj0 + lj;
       if (k < S::N4 && gi >= 0 && gi < a.nr && gj >= 0 && gj < a.nc) {
-        const int64_t idx = gi * a.nc + gj;
+        const int64_t idx = (int64_t)gi * a.nc + gj;
         L->h[r] = a.ih.ld(idx);
         if (li >= -3 && li < TH + 3 && lj >= -3 && lj < TW + 3) {
           L->z[r] = load_bed(a, gi, gj, idx, tdat);
@@ -590,7 +691,7 @@ template <class T, int TW, int TH> struct Stage {
     }
   }
   template <int NT>
-  static HF_HD void load_store(S& s, const StageArgs<T>& a, int64_t gi0, int64_t gj0, int tid, const LoadRegs<NT>& L) {
+  static HF_HD void load_store(S& s, const StageArgs<T>& a, int gi0, int gj0, int tid, const LoadRegs<NT>& L) {
     constexpr int RN = LoadRegs<NT>::RN;
     const T Z = cst<T>(0.0);
 #pragma unroll
@@ -598,7 +699,7 @@ template <class T, int TW, int TH> struct Stage {
       const int k = tid + r * NT;
       if (k >= S::N4) break;
       const int li = k / S::W4 - 4, lj = k % S::W4 - 4;
-      const int64_t gi = gi0 + li, gj = gj0 + lj;
+      const int gi = gi0 + li, gj = gj0 + lj;
       const bool inside = gi >= 0 && gi < a.nr && gj >= 0 && gj < a.nc;
       const bool in3 = li >= -3 && li < TH + 3 && lj >= -3 && lj < TW + 3;
       if (inside) {
@@ -629,7 +730,7 @@ template <class T, int TW, int TH> struct Stage {
     }
   }
   template <int NT>
-  static HF_HD void phase_load(S& s, const StageArgs<T>& a, int64_t gi0, int64_t gj0, int tid) {
+  static HF_HD void phase_load(S& s, const StageArgs<T>& a, int gi0, int gj0, int tid) {
     LoadRegs<NT> L;
     load_issue<NT>(a, gi0, gj0, tid, &L);
     load_store<NT>(s, a, gi0, gj0, tid, L);
@@ -639,14 +740,14 @@ template <class T, int TW, int TH> struct Stage {
   // (B - T)/dy, shared by the two cells whose minmod slopes use them
   // (update.rs:459-540 evaluates each twice; the values are identical), and
   // the near-dry flags of update.rs:433-444 (window clipped to the domain).
-  static HF_HD void phase_diffs(S& s, const StageArgs<T>& a, int64_t gi0, int64_t gj0, int tid, int nt) {
+  static HF_HD void phase_diffs(S& s, const StageArgs<T>& a, int gi0, int gj0, int tid, int nt) {
     const T Z = cst<T>(0.0);
     const R dx = R(a.dx), dy = R(a.dy);
     const int n_total = S::NDX + S::NDY + S::NND;
     for (int k = tid; k < n_total; k += nt) {
       if (k < S::NDX) {
         const int li = k / S::WDX - 1, fj = k % S::WDX - 2;
-        const int64_t gi = gi0 + li, gj = gj0 + fj;
+        const int gi = gi0 + li, gj = gj0 + fj;
         T de = Z, du = Z, dv = Z;
         if (gi >= 0 && gi < a.nr && gj >= 1 && gj < a.nc) {
           const int l = S::i3(li, fj - 1), r = S::i3(li, fj);
@@ -663,7 +764,7 @@ template <class T, int TW, int TH> struct Stage {
       } else if (k < S::NDX + S::NDY) {
         const int kk = k - S::NDX;
         const int fi = kk / S::WDY - 2, lj = kk % S::WDY - 1;
-        const int64_t gi = gi0 + fi, gj = gj0 + lj;
+        const int gi = gi0 + fi, gj = gj0 + lj;
         T de = Z, du = Z, dv = Z;
         if (gj >= 0 && gj < a.nc && gi >= 1 && gi < a.nr) {
           const int t = S::i3(fi - 1, lj), b = S::i3(fi, lj);
@@ -709,7 +810,7 @@ template <class T, int TW, int TH> struct Stage {
   // Minmod slope of cell (li, lj) along x from the face differences
   // (update.rs:459-499): zero near a dry cell or on a 1-column mesh,
   // one-sided on the first/last column, else minmod(left, right).
-  static HF_HD void slope_x(const S& s, const StageArgs<T>& a, int li, int lj, int64_t gj, T* se, T* su,
+  static HF_HD void slope_x(const S& s, const StageArgs<T>& a, int li, int lj, int gj, T* se, T* su,
                             T* sv) {
     const T Z = cst<T>(0.0);
     if constexpr (kSelectSlopes) {
@@ -741,7 +842,7 @@ template <class T, int TW, int TH> struct Stage {
     }
   }
   // update.rs:503-540
-  static HF_HD void slope_y(const S& s, const StageArgs<T>& a, int li, int lj, int64_t gi, T* se, T* su,
+  static HF_HD void slope_y(const S& s, const StageArgs<T>& a, int li, int lj, int gi, T* se, T* su,
                             T* sv) {
     const T Z = cst<T>(0.0);
     if constexpr (kSelectSlopes) {
@@ -777,7 +878,7 @@ template <class T, int TW, int TH> struct Stage {
   // reads the slopes of its two cells, and each cell has two faces per
   // axis), into the slope arrays; the update phase reads the eta slopes of
   // the tile's cells from them too.
-  static HF_HD void phase_slopes(S& s, const StageArgs<T>& a, int64_t gi0, int64_t gj0, int tid, int nt) {
+  static HF_HD void phase_slopes(S& s, const StageArgs<T>& a, int gi0, int gj0, int tid, int nt) {
     for (int k = tid; k < S::NSX + S::NSY; k += nt) {
       T e, u, v;
       if (k < S::NSX) {
@@ -799,9 +900,9 @@ template <class T, int TW, int TH> struct Stage {
 
   // Raw state of a domain cell (boundary faces use it unreconstructed,
   // update.rs:892-897).
-  static HF_HD void raw(const S& s, const StageArgs<T>& a, int li, int lj, int64_t gi, int64_t gj, T* h,
+  static HF_HD void raw(const S& s, const StageArgs<T>& a, int li, int lj, int gi, int gj, T* h,
                         T* hu, T* hv) {
-    const int64_t idx = gi * a.nc + gj;
+    const int64_t idx = (int64_t)gi * a.nc + gj;
     *h = s.h[S::i3(li, lj)];
     *hu = a.ihu.ld(idx);
     *hv = a.ihv.ld(idx);
@@ -809,10 +910,10 @@ template <class T, int TW, int TH> struct Stage {
 
   // One x-face (local row li, face column fj) into the face arrays
   // (update.rs:560-588, 890-905).
-  static HF_HD void face_x(S& s, const StageArgs<T>& a, int64_t gi0, int64_t gj0, int k) {
+  static HF_HD void face_x(S& s, const StageArgs<T>& a, int gi0, int gj0, int k) {
     const T Z = cst<T>(0.0);
     const int li = k / S::WFX - 1, fj = k % S::WFX - 1;
-    const int64_t gi = gi0 + li, gjf = gj0 + fj;
+    const int gi = gi0 + li, gjf = gj0 + fj;
     T m = Z, na = Z, nb = Z, tg = Z;
     if (gi >= 0 && gi < a.nr && gjf >= 0 && gjf <= a.nc) {
       T zf = zface_x(s, a, gi0, gj0, li, fj);
@@ -820,14 +921,14 @@ template <class T, int TW, int TH> struct Stage {
         const bool west = (gjf == 0);
         const int lc = west ? fj : fj - 1;
         const int ln = west ? fj + 1 : fj - 2;
-        const int64_t gc = west ? 0 : a.nc - 1;
+        const int gc = west ? 0 : a.nc - 1;
         T h, hu, hv, gh, ghu, ghv;
         raw(s, a, li, lc, gi, gc, &h, &hu, &hv);
         T zc = s.z[S::i3(li, lc)];
         T zn = (a.nc >= 2) ? s.z[S::i3(li, ln)] : zc;
         const int side = west ? SIDE_W : SIDE_E;
         ghost_state(a.bc_kind[side], a.bc_param[side], side, h, hu, hv, zc, zn, a.nc >= 2, a.dx,
-                    a.man.ld(gi * a.nc + gc), &gh, &ghu, &ghv);
+                    a.man.ld((int64_t)gi * a.nc + gc), &gh, &ghu, &ghv);
         if (kRel) zf = Z;
         if (west)
           well_balanced_face(gh, ghu, ghv, h, hu, hv, zf, &m, &na, &nb, &tg);
@@ -866,10 +967,10 @@ template <class T, int TW, int TH> struct Stage {
   }
 
   // One y-face (face row fi, local column lj) (update.rs:595-623, 907-922).
-  static HF_HD void face_y(S& s, const StageArgs<T>& a, int64_t gi0, int64_t gj0, int k) {
+  static HF_HD void face_y(S& s, const StageArgs<T>& a, int gi0, int gj0, int k) {
     const T Z = cst<T>(0.0);
     const int fi = k / S::WFY - 1, lj = k % S::WFY - 1;
-    const int64_t gif = gi0 + fi, gj = gj0 + lj;
+    const int gif = gi0 + fi, gj = gj0 + lj;
     T m = Z, na = Z, nb = Z, tg = Z;
     if (gj >= 0 && gj < a.nc && gif >= 0 && gif <= a.nr) {
       T zf = zface_y(s, a, gi0, gj0, fi, lj);
@@ -877,14 +978,14 @@ template <class T, int TW, int TH> struct Stage {
         const bool north = (gif == 0);
         const int lc = north ? fi : fi - 1;
         const int ln = north ? fi + 1 : fi - 2;
-        const int64_t gc = north ? 0 : a.nr - 1;
+        const int gc = north ? 0 : a.nr - 1;
         T h, hu, hv, gh, ghu, ghv;
         raw(s, a, lc, lj, gc, gj, &h, &hu, &hv);
         T zc = s.z[S::i3(lc, lj)];
         T zn = (a.nr >= 2) ? s.z[S::i3(ln, lj)] : zc;
         const int side = north ? SIDE_N : SIDE_S;
         ghost_state(a.bc_kind[side], a.bc_param[side], side, h, hu, hv, zc, zn, a.nr >= 2, a.dy,
-                    a.man.ld(gc * a.nc + gj), &gh, &ghu, &ghv);
+                    a.man.ld((int64_t)gc * a.nc + gj), &gh, &ghu, &ghv);
         // y-face: normal momentum is hv, tangential hu (riemann.rs:97-107)
         if (kRel) zf = Z;
         if (north)
@@ -930,7 +1031,7 @@ template <class T, int TW, int TH> struct Stage {
   // and tangential velocity and slopes) and the stride between its two
   // cells, so no wavefront runs two face kinds. Faces outside the domain
   // get zero fluxes; domain-edge faces are left to boundary_faces.
-  static HF_HD void face_interior(S& s, const StageArgs<T>& a, int64_t gi0, int64_t gj0, int k) {
+  static HF_HD void face_interior(S& s, const StageArgs<T>& a, int gi0, int gj0, int k) {
     const T Z = cst<T>(0.0);
     const bool ya = k >= S::NFX;
     const int kx = k, ky = ya ? k - S::NFX : 0;
@@ -938,9 +1039,9 @@ template <class T, int TW, int TH> struct Stage {
     // local cell right of an x-face / below a y-face
     const int li = ya ? ry - 1 : rx - 1;
     const int lj = ya ? ky - ry * S::WFY - 1 : kx - rx * S::WFX - 1;
-    const int64_t gi = gi0 + li, gj = gj0 + lj;
-    const int64_t g_norm = ya ? gi : gj, n_norm = ya ? a.nr : a.nc;  // along the face normal
-    const int64_t g_tan = ya ? gj : gi, n_tan = ya ? a.nc : a.nr;
+    const int gi = gi0 + li, gj = gj0 + lj;
+    const int g_norm = ya ? gi : gj, n_norm = ya ? a.nr : a.nc;  // along the face normal
+    const int g_tan = ya ? gj : gi, n_tan = ya ? a.nc : a.nr;
     const bool inside = g_tan >= 0 && g_tan < n_tan && g_norm >= 0 && g_norm <= n_norm;
     const bool edge = g_norm == 0 || g_norm == n_norm;
     if (inside && edge) return;  // boundary_faces
@@ -975,7 +1076,12 @@ template <class T, int TW, int TH> struct Stage {
           h_m = rmax(eta_m - zf, Z);
           h_p = rmax(eta_p - zf, Z);
         }
-        well_balanced_face(h_m, h_m * n_m, h_m * t_m, h_p, h_p * n_p, h_p * t_p, zf, &m, &na, &nb, &tg);
+        if constexpr (PrimFace<T>::on) {
+          prim_face(h_m, n_m, t_m, h_p, n_p, t_p, &m, &na, &tg);
+          nb = na;
+        } else {
+          well_balanced_face(h_m, h_m * n_m, h_m * t_m, h_p, h_p * n_p, h_p * t_p, zf, &m, &na, &nb, &tg);
+        }
       }
     }
     const int kk = ya ? ky : kx;
@@ -992,20 +1098,20 @@ template <class T, int TW, int TH> struct Stage {
   // The domain-edge faces in the tile's face region (ghost cells,
   // boundary.rs:193-367): x-faces in the west (column 0) and east (column
   // nc) edge columns, y-faces in the north and south edge rows.
-  static HF_HD void boundary_faces(S& s, const StageArgs<T>& a, int64_t gi0, int64_t gj0, int tid, int nt) {
+  static HF_HD void boundary_faces(S& s, const StageArgs<T>& a, int gi0, int gj0, int tid, int nt) {
     constexpr int NXB = 2 * (TH + 2), NYB = 2 * (TW + 2);
     for (int t = tid; t < NXB + NYB; t += nt) {
       if (t < NXB) {
         const int li = t % (TH + 2) - 1;
-        const int64_t fj = (t < TH + 2) ? -gj0 : a.nc - gj0;
-        const int64_t gi = gi0 + li;
+        const int fj = (t < TH + 2) ? -gj0 : a.nc - gj0;
+        const int gi = gi0 + li;
         if (fj < -1 || fj > TW + 1 || gi < 0 || gi >= a.nr) continue;
         face_x(s, a, gi0, gj0, (li + 1) * S::WFX + (int)fj + 1);
       } else {
         const int u = t - NXB;
         const int lj = u % (TW + 2) - 1;
-        const int64_t fi = (u < TW + 2) ? -gi0 : a.nr - gi0;
-        const int64_t gj = gj0 + lj;
+        const int fi = (u < TW + 2) ? -gi0 : a.nr - gi0;
+        const int gj = gj0 + lj;
         if (fi < -1 || fi > TH + 1 || gj < 0 || gj >= a.nc) continue;
         face_y(s, a, gi0, gj0, ((int)fi + 1) * S::WFY + lj + 1);
       }
@@ -1015,7 +1121,7 @@ template <class T, int TW, int TH> struct Stage {
   // Phase 4: every x- and y-face of the tile and its 1-cell ring: interior
   // faces in one loop over both axes, then, in tiles that reach the domain
   // edge, the edge faces.
-  static HF_HD void phase_faces(S& s, const StageArgs<T>& a, int64_t gi0, int64_t gj0, int tid, int nt) {
+  static HF_HD void phase_faces(S& s, const StageArgs<T>& a, int gi0, int gj0, int tid, int nt) {
     for (int k = tid; k < S::NFX + S::NFY; k += nt) face_interior(s, a, gi0, gj0, k);
     const bool near_edge = gi0 <= 1 || gj0 <= 1 || gi0 + TH + 1 >= a.nr || gj0 + TW + 1 >= a.nc;
     if (near_edge) boundary_faces(s, a, gi0, gj0, tid, nt);
@@ -1023,13 +1129,13 @@ template <class T, int TW, int TH> struct Stage {
 
   // Phase 5: Liang-Marche outflow limiter alpha on the tile and its ring
   // (update.rs:953-982).
-  static HF_HD void phase_alpha(S& s, const StageArgs<T>& a, int64_t gi0, int64_t gj0, int tid, int nt,
+  static HF_HD void phase_alpha(S& s, const StageArgs<T>& a, int gi0, int gj0, int tid, int nt,
                                 double dt) {
     const R dt_dx = R(dt / a.dx), dt_dy = R(dt / a.dy);
     const T Z = cst<T>(0.0);
     for (int k = tid; k < S::NA; k += nt) {
       const int li = k / S::WA - 1, lj = k % S::WA - 1;
-      const int64_t gi = gi0 + li, gj = gj0 + lj;
+      const int gi = gi0 + li, gj = gj0 + lj;
       T al = cst<T>(1.0);
       if (gi >= 0 && gi < a.nr && gj >= 0 && gj < a.nc) {
         T fx_right = s.w.f.fxm[S::ifx(li, lj + 1)];
@@ -1050,14 +1156,14 @@ template <class T, int TW, int TH> struct Stage {
 
   // Upstream-alpha scaling of one x-face as seen from local row li
   // (update.rs:992-1025). Dry-dry interior faces are left as computed.
-  static HF_HD void scaled_fx(const S& s, const StageArgs<T>& a, int64_t gj0, int li, int fj, T* m, T* na,
+  static HF_HD void scaled_fx(const S& s, const StageArgs<T>& a, int gj0, int li, int fj, T* m, T* na,
                               T* nb, T* tg) {
     const int k = S::ifx(li, fj);
     *m = s.w.f.fxm[k];
     *na = s.w.f.fxa[k];
     *nb = s.w.f.fxb[k];
     *tg = s.w.f.fxt[k];
-    const int64_t gjf = gj0 + fj;
+    const int gjf = gj0 + fj;
     // both upstream candidates read with the face, not after its sign is known
     const T al_l = s.alpha[S::ia(li, fj - 1)], al_r = s.alpha[S::ia(li, fj)];
     if ((s.dry[S::i4(li, fj - 1)] & s.dry[S::i4(li, fj)]) != 0 && gjf > 0 && gjf < a.nc) return;
@@ -1074,14 +1180,14 @@ template <class T, int TW, int TH> struct Stage {
     *tg = *tg * al;
   }
   // update.rs:1026-1053
-  static HF_HD void scaled_fy(const S& s, const StageArgs<T>& a, int64_t gi0, int fi, int lj, T* m, T* na,
+  static HF_HD void scaled_fy(const S& s, const StageArgs<T>& a, int gi0, int fi, int lj, T* m, T* na,
                               T* nb, T* tg) {
     const int k = S::ify(fi, lj);
     *m = s.w.f.fym[k];
     *na = s.w.f.fya[k];
     *nb = s.w.f.fyb[k];
     *tg = s.w.f.fyt[k];
-    const int64_t gif = gi0 + fi;
+    const int gif = gi0 + fi;
     const T al_t = s.alpha[S::ia(fi - 1, lj)], al_b = s.alpha[S::ia(fi, lj)];
     if ((s.dry[S::i4(fi - 1, lj)] & s.dry[S::i4(fi, lj)]) != 0 && gif > 0 && gif < a.nr) return;
     T al;
@@ -1099,7 +1205,7 @@ template <class T, int TW, int TH> struct Stage {
 
   // Wave speeds of one cell for the CFL bound (update.rs:113-136) and, on
   // domain edges, of its ghosts (update.rs:179-201, no H_VEL cut there).
-  static HF_HD void cell_wave_speed(const StageArgs<T>& a, int64_t gi, int64_t gj, T h, T hu, T hv, T zc,
+  static HF_HD void cell_wave_speed(const StageArgs<T>& a, int gi, int gj, T h, T hu, T hv, T zc,
                                     const T* zn4, const T* man, double* sx, double* sy) {
     const R hv_ = val(h);
     if (hv_ > R(H_DRY)) {
@@ -1215,10 +1321,10 @@ template <class T, int TW, int TH> struct Stage {
     T man;           // Manning n (final stage: friction, Discharge ghosts)
   };
   template <int MODE>
-  static HF_HD void prefetch(const StageArgs<T>& a, int64_t gi0, int64_t gj0, int tid, Pre* p) {
-    const int64_t gi = gi0 + tid / TW, gj = gj0 + tid % TW;
+  static HF_HD void prefetch(const StageArgs<T>& a, int gi0, int gj0, int tid, Pre* p) {
+    const int gi = gi0 + tid / TW, gj = gj0 + tid % TW;
     if (tid >= TW * TH || gi >= a.nr || gj >= a.nc) return;
-    const int64_t idx = gi * a.nc + gj;
+    const int64_t idx = (int64_t)gi * a.nc + gj;
     p->hu = a.ihu.ld(idx);
     p->hv = a.ihv.ld(idx);
     if (MODE == MODE_FINAL_RK2) {
@@ -1235,7 +1341,7 @@ template <class T, int TW, int TH> struct Stage {
   // *wet is set when an output cell is wet (not h <= H_DRY). pre: the
   // prefetched values of cell k = tid (null: read them here).
   template <int MODE>
-  static HF_HD void phase_update(S& s, const StageArgs<T>& a, int64_t gi0, int64_t gj0, int tid, int nt,
+  static HF_HD void phase_update(S& s, const StageArgs<T>& a, int gi0, int gj0, int tid, int nt,
                                  double dt, double* wsx, double* wsy, bool* wet, const Pre* pre = nullptr) {
     const T Z = cst<T>(0.0);
     const R dt_dx = R(dt / a.dx), dt_dy = R(dt / a.dy), rdt = R(dt);
@@ -1243,9 +1349,9 @@ template <class T, int TW, int TH> struct Stage {
     if (MODE == MODE_FINAL_RK2) fnow = forcing_now(a, gi0, gj0, TH, TW);
     for (int k = tid; k < TW * TH; k += nt) {
       const int li = k / TW, lj = k % TW;
-      const int64_t gi = gi0 + li, gj = gj0 + lj;
+      const int gi = gi0 + li, gj = gj0 + lj;
       if (gi >= a.nr || gj >= a.nc) continue;
-      const int64_t idx = gi * a.nc + gj;
+      const int64_t idx = (int64_t)gi * a.nc + gj;
       const Pre* pk = (pre && k == tid) ? pre : nullptr;
       const int c3 = S::i3(li, lj);
       const T h_old = s.h[c3];

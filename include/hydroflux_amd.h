@@ -119,8 +119,13 @@ typedef struct hf2d_config {
 
 typedef struct hf2d_sim hf2d_sim;
 
+/* Largest grid side (rows or columns) plus one: cell coordinates are 32-bit
+ * on the device. */
+#define HF2D_MAX_SIDE (1LL << 30)
+
 /* Mesh2DG::new / StepWorkspace2D::new (geometry.rs:58-76, update.rs:756-771).
- * Fails with HF2D_INVALID_CONFIG on dx <= 0, dy <= 0 or an empty grid. */
+ * Fails with HF2D_INVALID_CONFIG on dx <= 0, dy <= 0, an empty grid or a
+ * side of HF2D_MAX_SIDE cells or more. */
 int hf2d_create(int64_t n_rows, int64_t n_cols, double dx, double dy, int32_t scalar_kind,
                 hf2d_sim** out);
 int hf2d_destroy(hf2d_sim* sim);

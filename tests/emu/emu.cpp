@@ -151,8 +151,8 @@ template <class T> void tail(const StageArgs<T>& a, unsigned long long* dst) {
   if (!stage_prologue(a, true, &dt)) return;
   double wx = 0.0, wy = 0.0;
   const typename St::ForcingNow fnow = St::forcing_now(a, 0, 0, a.nr, a.nc);
-  for (int64_t idx = 0; idx < a.nr * a.nc; ++idx) {
-    const int64_t gi = idx / a.nc, gj = idx % a.nc;
+  for (int64_t idx = 0; idx < (int64_t)a.nr * a.nc; ++idx) {
+    const int gi = (int)(idx / a.nc), gj = (int)(idx % a.nc);
     T h = a.ih.ld(idx), hu = a.ihu.ld(idx), hv = a.ihv.ld(idx);
     St::forcing(a, fnow, gi, gj, dt, &h, &hu, &hv);
     a.oh.st(idx, h);
