@@ -3,8 +3,8 @@
 A Batch holds many independent flood scenarios of different grid sizes in
 one HBM arena. Each scenario runs to its own end time with its own dt
 (Simulation::run_until semantics, sim.rs:231-246, plus the configured rain
-and point sources); all of them advance together, one persistent launch per
-RK stage, the step replayed from a hipGraph.
+and point sources); all of them advance together, one launch per RK stage
+over all their tiles, the step replayed from a hipGraph.
 """
 from __future__ import annotations
 

@@ -3,8 +3,9 @@
 //
 // Memory: one HBM arena per batch, carved by a first-fit allocator; a
 // scenario is admitted only if its footprint fits (hf2d_batch_footprint).
-// Per step: one persistent launch per RK stage over a device work list of
-// (scenario, tile) items, then one finalize launch over the active slots;
+// Per step: one launch per RK stage over a device work list of (scenario,
+// tile) items, one workgroup per item, then one finalize launch over the
+// active slots;
 // the step sequence is captured in a hipGraph and replayed, and re-captured
 // only when the set of scenarios or the tile configuration changes. The tile
 // shape / threads per workgroup is chosen by hf2d_batch_autotune, which
@@ -35,9 +36,9 @@ struct CfgInfo {
 // The autotuner's candidates per kind: the 256-cell tile shapes
 // (16x16, 32x8, 64x4, 128x2) and the larger high-occupancy tiles.
 const CfgInfo CFG_F32[] = {{32, 32, 1024}, {32, 16, 512}, {16, 16, 256}, {32, 8, 256}, {64, 4, 256}, {128, 2, 256}};
-// FP64 32 x 24 with 1024 threads is limited to 128 VGPRs and spills in the
-// persistent batch loop; 768 threads (170 VGPRs) does not.
-const CfgInfo CFG_F64[] = {{32, 24, 768}, {32, 24, 1024}, {32, 16, 1024}, {16, 16, 256}, {32, 8, 256}};
+// 32 x 12 with 512 threads holds 79 KB of LDS: two workgroups per CU.
+const CfgInfo CFG_F64[] = {{32, 24, 768}, {32, 24, 1024}, {32, 16, 1024}, {32, 12, 512},
+                           {16, 16, 256}, {32, 8, 256},   {64, 4, 256},   {128, 2, 256}};
 const CfgInfo CFG_DUAL[] = {{16, 8, 512}, {16, 16, 1024}, {16, 8, 256}};
 
 template <class T> struct Cfgs;
@@ -47,7 +48,7 @@ template <> struct Cfgs<float> {
 };
 template <> struct Cfgs<double> {
   static const CfgInfo* list() { return CFG_F64; }
-  static constexpr int n = 5;
+  static constexpr int n = 8;
 };
 template <> struct Cfgs<dual> {
   static const CfgInfo* list() { return CFG_DUAL; }
@@ -55,78 +56,46 @@ template <> struct Cfgs<dual> {
 };
 
 template <class T, int TW, int TH, int NT>
-int launch_pair(hipStream_t st, const StageArgs<T>* d1, const StageArgs<T>* d2, const WorkItem* w, int n, int grid) {
-  hipLaunchKernelGGL((batch_stage_kernel<T, TW, TH, NT, hf::MODE_STAGE>), dim3(grid), dim3(NT), 0, st, d1, w, n);
-  hipLaunchKernelGGL((batch_stage_kernel<T, TW, TH, NT, hf::MODE_FINAL_RK2>), dim3(grid), dim3(NT), 0, st, d2, w,
-                     n);
+int launch_pair(hipStream_t st, const StageArgs<T>* d1, const StageArgs<T>* d2, const WorkItem* w, int n) {
+  hipLaunchKernelGGL((batch_stage_kernel<T, TW, TH, NT, hf::MODE_STAGE>), dim3(n), dim3(NT), 0, st, d1, w, n);
+  hipLaunchKernelGGL((batch_stage_kernel<T, TW, TH, NT, hf::MODE_FINAL_RK2>), dim3(n), dim3(NT), 0, st, d2, w, n);
   return hipGetLastError() == hipSuccess ? 0 : 1;
-}
-template <class T, int TW, int TH, int NT> int occupancy_blocks() {
-  int nb = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch_stage_kernel<T, TW, TH, NT, hf::MODE_FINAL_RK2>, NT,
-                                                   0) != hipSuccess)
-    return 1;
-  return nb > 0 ? nb : 1;
 }
 
 // Dispatch on the configuration index of kind T.
 template <class T> int launch_cfg(int ci, hipStream_t st, const StageArgs<T>* d1, const StageArgs<T>* d2,
-                                  const WorkItem* w, int n, int grid);
-template <class T> int occupancy_cfg(int ci);
+                                  const WorkItem* w, int n);
 
 template <> int launch_cfg<float>(int ci, hipStream_t st, const StageArgs<float>* d1, const StageArgs<float>* d2,
-                                  const WorkItem* w, int n, int g) {
+                                  const WorkItem* w, int n) {
   switch (ci) {
-    case 0: return launch_pair<float, 32, 32, 1024>(st, d1, d2, w, n, g);
-    case 1: return launch_pair<float, 32, 16, 512>(st, d1, d2, w, n, g);
-    case 2: return launch_pair<float, 16, 16, 256>(st, d1, d2, w, n, g);
-    case 3: return launch_pair<float, 32, 8, 256>(st, d1, d2, w, n, g);
-    case 4: return launch_pair<float, 64, 4, 256>(st, d1, d2, w, n, g);
-    default: return launch_pair<float, 128, 2, 256>(st, d1, d2, w, n, g);
-  }
-}
-template <> int occupancy_cfg<float>(int ci) {
-  switch (ci) {
-    case 0: return occupancy_blocks<float, 32, 32, 1024>();
-    case 1: return occupancy_blocks<float, 32, 16, 512>();
-    case 2: return occupancy_blocks<float, 16, 16, 256>();
-    case 3: return occupancy_blocks<float, 32, 8, 256>();
-    case 4: return occupancy_blocks<float, 64, 4, 256>();
-    default: return occupancy_blocks<float, 128, 2, 256>();
+    case 0: return launch_pair<float, 32, 32, 1024>(st, d1, d2, w, n);
+    case 1: return launch_pair<float, 32, 16, 512>(st, d1, d2, w, n);
+    case 2: return launch_pair<float, 16, 16, 256>(st, d1, d2, w, n);
+    case 3: return launch_pair<float, 32, 8, 256>(st, d1, d2, w, n);
+    case 4: return launch_pair<float, 64, 4, 256>(st, d1, d2, w, n);
+    default: return launch_pair<float, 128, 2, 256>(st, d1, d2, w, n);
   }
 }
 template <> int launch_cfg<double>(int ci, hipStream_t st, const StageArgs<double>* d1, const StageArgs<double>* d2,
-                                   const WorkItem* w, int n, int g) {
+                                   const WorkItem* w, int n) {
   switch (ci) {
-    case 0: return launch_pair<double, 32, 24, 768>(st, d1, d2, w, n, g);
-    case 1: return launch_pair<double, 32, 24, 1024>(st, d1, d2, w, n, g);
-    case 2: return launch_pair<double, 32, 16, 1024>(st, d1, d2, w, n, g);
-    case 3: return launch_pair<double, 16, 16, 256>(st, d1, d2, w, n, g);
-    default: return launch_pair<double, 32, 8, 256>(st, d1, d2, w, n, g);
-  }
-}
-template <> int occupancy_cfg<double>(int ci) {
-  switch (ci) {
-    case 0: return occupancy_blocks<double, 32, 24, 768>();
-    case 1: return occupancy_blocks<double, 32, 24, 1024>();
-    case 2: return occupancy_blocks<double, 32, 16, 1024>();
-    case 3: return occupancy_blocks<double, 16, 16, 256>();
-    default: return occupancy_blocks<double, 32, 8, 256>();
+    case 0: return launch_pair<double, 32, 24, 768>(st, d1, d2, w, n);
+    case 1: return launch_pair<double, 32, 24, 1024>(st, d1, d2, w, n);
+    case 2: return launch_pair<double, 32, 16, 1024>(st, d1, d2, w, n);
+    case 3: return launch_pair<double, 32, 12, 512>(st, d1, d2, w, n);
+    case 4: return launch_pair<double, 16, 16, 256>(st, d1, d2, w, n);
+    case 5: return launch_pair<double, 32, 8, 256>(st, d1, d2, w, n);
+    case 6: return launch_pair<double, 64, 4, 256>(st, d1, d2, w, n);
+    default: return launch_pair<double, 128, 2, 256>(st, d1, d2, w, n);
   }
 }
 template <> int launch_cfg<dual>(int ci, hipStream_t st, const StageArgs<dual>* d1, const StageArgs<dual>* d2,
-                                 const WorkItem* w, int n, int g) {
+                                 const WorkItem* w, int n) {
   switch (ci) {
-    case 0: return launch_pair<dual, 16, 8, 512>(st, d1, d2, w, n, g);
-    case 1: return launch_pair<dual, 16, 16, 1024>(st, d1, d2, w, n, g);
-    default: return launch_pair<dual, 16, 8, 256>(st, d1, d2, w, n, g);
-  }
-}
-template <> int occupancy_cfg<dual>(int ci) {
-  switch (ci) {
-    case 0: return occupancy_blocks<dual, 16, 8, 512>();
-    case 1: return occupancy_blocks<dual, 16, 16, 1024>();
-    default: return occupancy_blocks<dual, 16, 8, 256>();
+    case 0: return launch_pair<dual, 16, 8, 512>(st, d1, d2, w, n);
+    case 1: return launch_pair<dual, 16, 16, 1024>(st, d1, d2, w, n);
+    default: return launch_pair<dual, 16, 8, 256>(st, d1, d2, w, n);
   }
 }
 
@@ -185,8 +154,8 @@ struct hf2d_batch {
   WorkItem* work = nullptr;
   int work_cap = 0, n_work = 0;
   int* slots_dev = nullptr;
+  unsigned long long* ws_part = nullptr;  // BATCH_WS_SLOTS CFL slots per scenario slot
   int n_active = 0;
-  int grid = 1, n_cu = 256;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   int graph_steps = 0;
@@ -290,18 +259,18 @@ void destroy_graph(hf2d_batch* b) {
 template <class T> int launch_step(hf2d_batch* b) {
   if (b->n_work > 0) {
     if (launch_cfg<T>(b->cfgi, b->stream, (const StageArgs<T>*)b->descs1, (const StageArgs<T>*)b->descs2, b->work,
-                      b->n_work, b->grid))
+                      b->n_work))
       return bfail(b, HF2D_HIP_ERROR, "stage launch failed");
   }
   if (b->n_active > 0) {
-    hipLaunchKernelGGL(batch_finalize_kernel, dim3((unsigned)((b->n_active + 255) / 256)), dim3(256), 0, b->stream,
-                       b->ctrls, b->slots_dev, b->n_active);
+    hipLaunchKernelGGL(batch_finalize_kernel, dim3((unsigned)b->n_active), dim3(BATCH_WS_SLOTS), 0, b->stream,
+                       b->ctrls, b->slots_dev, b->n_active, b->ws_part);
     BCHK(b, hipGetLastError());
   }
   return HF2D_OK;
 }
 
-// Rebuild descriptors, work list, persistent grid and the step graph.
+// Rebuild descriptors, work list and the step graph.
 template <class T> int rebuild(hf2d_batch* b) {
   const CfgInfo ci = Cfgs<T>::list()[b->cfgi];
   std::vector<StageArgs<T>> d1(b->sc.size()), d2(b->sc.size());
@@ -316,6 +285,7 @@ template <class T> int rebuild(hf2d_batch* b) {
     c.dt_mode = 1;
     c.first_stage = 0;
     c.wave_speed = 1;
+    c.ws_part = b->ws_part + k * BATCH_WS_SLOTS * WS_STRIDE;
     d1[k] = a;
     d2[k] = c;
     const int nt = a.tiles_x * a.tiles_y;
@@ -337,7 +307,6 @@ template <class T> int rebuild(hf2d_batch* b) {
   if (!slots.empty()) BCHK(b, hipMemcpy(b->slots_dev, slots.data(), slots.size() * sizeof(int), hipMemcpyHostToDevice));
   b->n_work = (int)w.size();
   b->n_active = (int)slots.size();
-  b->grid = std::max(1, std::min(b->n_work, occupancy_cfg<T>(b->cfgi) * b->n_cu));
   destroy_graph(b);
   b->dirty = false;
   return HF2D_OK;
@@ -477,7 +446,7 @@ template <class T> int autotune(hf2d_batch* b, int steps, int32_t* best_out, dou
 extern "C" {
 
 int hf2d_batch_create(int32_t scalar_kind, int64_t capacity_bytes, int32_t max_slots, hf2d_batch** out) {
-  if (!out || capacity_bytes <= 0 || max_slots <= 0) return HF2D_INVALID_ARGUMENT;
+  if (!out || capacity_bytes <= 0 || max_slots <= 0 || max_slots > HF2D_BATCH_MAX_SLOTS) return HF2D_INVALID_ARGUMENT;
   if (scalar_kind != HF2D_F64 && scalar_kind != HF2D_DUAL && scalar_kind != HF2D_F32) return HF2D_INVALID_ARGUMENT;
   *out = nullptr;
   hf2d_batch* b = new hf2d_batch();
@@ -490,7 +459,6 @@ int hf2d_batch_create(int32_t scalar_kind, int64_t capacity_bytes, int32_t max_s
   };
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess) return bad();
-  if (hipDeviceGetAttribute(&b->n_cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return bad();
   if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) return bad();
   if (hipMalloc(&b->arena, b->cap) != hipSuccess) return bad();
   b->free_blocks[0] = b->cap;
@@ -500,6 +468,9 @@ int hf2d_batch_create(int32_t scalar_kind, int64_t capacity_bytes, int32_t max_s
   if (hipMalloc(&b->descs1, (size_t)max_slots * ds) != hipSuccess) return bad();
   if (hipMalloc(&b->descs2, (size_t)max_slots * ds) != hipSuccess) return bad();
   if (hipMalloc(&b->slots_dev, (size_t)max_slots * sizeof(int)) != hipSuccess) return bad();
+  const size_t wsb = (size_t)max_slots * BATCH_WS_SLOTS * WS_STRIDE * sizeof(unsigned long long);
+  if (hipMalloc(&b->ws_part, wsb) != hipSuccess) return bad();
+  if (hipMemset(b->ws_part, 0, wsb) != hipSuccess) return bad();
   *out = b;
   return HF2D_OK;
 }
@@ -508,7 +479,7 @@ int hf2d_batch_destroy(hf2d_batch* b) {
   if (!b) return HF2D_OK;
   if (b->stream) (void)hipStreamSynchronize(b->stream);
   destroy_graph(b);
-  void* bufs[] = {b->arena, b->ctrls, b->descs1, b->descs2, b->work, b->slots_dev};
+  void* bufs[] = {b->arena, b->ctrls, b->descs1, b->descs2, b->work, b->slots_dev, b->ws_part};
   for (void* p : bufs)
     if (p) (void)hipFree(p);
   if (b->stream) (void)hipStreamDestroy(b->stream);

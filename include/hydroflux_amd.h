@@ -253,12 +253,16 @@ int hf2d_math_probe(const double* x, const double* y, int64_t n, double* out);
 /* ------------------------------------------------------------------ batches
  * Many independent scenarios (grids of different sizes, each with its own
  * clock, dt, boundaries and forcing) resident in one HBM arena and advanced
- * together: one persistent launch per RK stage over all their tiles, the
- * step captured in a hipGraph. No counterpart in the reference, which runs
- * one Simulation at a time; each scenario's arithmetic is that of
+ * together: one launch per RK stage over all their tiles (one workgroup per
+ * tile), the step captured in a hipGraph. No counterpart in the reference,
+ * which runs one Simulation at a time; each scenario's arithmetic is that of
  * hf2d_run_until (sim.rs:231-246) with the configured forcing. */
 typedef struct hf2d_batch hf2d_batch;
 
+/* Most scenario slots of one batch (each slot holds 8 KiB of CFL slots). */
+#define HF2D_BATCH_MAX_SLOTS 16384
+
+/* HF2D_INVALID_ARGUMENT unless 0 < max_slots <= HF2D_BATCH_MAX_SLOTS. */
 int hf2d_batch_create(int32_t scalar_kind, int64_t capacity_bytes, int32_t max_slots, hf2d_batch** out);
 int hf2d_batch_destroy(hf2d_batch* batch);
 const char* hf2d_batch_last_error(const hf2d_batch* batch);

@@ -89,6 +89,37 @@ def test_batch_f64_equals_single_grid_driver_and_oracle():
             assert np.array_equal(sim.get_state(), out)
 
 
+def test_batch_f64_every_tile_config_equals_single_grid():
+    """FP64 results do not depend on the tile shape: every configuration the
+    autotuner may pick (32x24 ... 128x2) gives the single-grid driver's bits."""
+    _say("batch f64 every tile config vs single grid")
+    from hydroflux_amd import Grid2D
+    from hydroflux_amd.batch import Batch
+
+    scs = _mix(2, 5)
+    specs = [_spec(sc, 12.0) for sc in scs]
+    ref = []
+    for sc in scs:
+        g = Grid2D(sc.n_rows, sc.n_cols, sc.dx, sc.dx)
+        g.set_mesh(sc.bed, sc.manning)
+        g.set_state(sc.dry_state())
+        g.set_boundaries(list(sc.bc_kinds))
+        g.configure(cfl=0.4, max_dt=1.0, rain_rate=sc.rain_rate, sources=sc.sources)
+        g.run_until(12.0)
+        ref.append((g.get_state(), g.time()))
+        g.close()
+    probe = Batch("f64", 1 << 26, 4)
+    shapes = probe.configs()
+    probe.close()
+    assert {(64, 4), (128, 2), (16, 16), (32, 8)} <= {(w, h) for w, h, _ in shapes}
+    for cfg, shape in enumerate(shapes):
+        _say(f"batch f64 config {cfg} {shape}")
+        res = _run_batch("f64", specs, config=cfg)
+        for (out, t, steps, status), (st, (t_ref, n_ref)) in zip(res, ref):
+            assert status == 0 and (t, steps) == (t_ref, n_ref), cfg
+            assert np.array_equal(out, st), cfg
+
+
 def test_batch_f32_alone_equals_in_mix_and_across_tile_configs():
     _say("batch f32 alone vs mix vs configs")
     scs = _mix(6, 11)
