@@ -113,9 +113,12 @@ def time_steps(g, warmup, steps):
     return (t1 - t0), s1 / max(cnt, 1), s2 / max(cnt, 1)
 
 
-# FP64 / FP32 vector issue of MI355X: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz
-# (packed FP32 is not used: one VALU instruction per lane per clock).
-VALU_PEAK_LANE_OPS = 256 * 4 * 16 * 2.4e9
+# Vector issue peaks of MI355X from its published vector rates, one FMA
+# (2 flops) per lane-instruction: FP64 78.6 TFLOP/s -> 39.3 T lane-ops/s
+# (256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz); FP32 157.3 TFLOP/s -> 78.6 T.
+# The FP32 form's kernels issue above 39.3 T lane-instructions/s (no packed
+# FP32 instructions are emitted), so the FP64 figure does not bound them.
+VALU_PEAK_LANE_OPS = {"f64": 78.6e12 / 2, "f32": 157.3e12 / 2}
 
 
 def _counters(key):
@@ -152,8 +155,9 @@ def roofline(kind, cells, t_first_ms, t_final_ms, traffic, kernel, counter_key=N
     if c:
         lane = c["valu_wave_instr_per_launch"] * 64.0
         issue = lane / (t_final_ms / 1e3)
+        peak = VALU_PEAK_LANE_OPS["f32" if kind == "f32" else "f64"]
         out["valu"] = {"lane_instr_per_cell": lane / cells, "achieved_lane_ops_per_s": issue,
-                       "peak_lane_ops_per_s": VALU_PEAK_LANE_OPS, "issue_frac": issue / VALU_PEAK_LANE_OPS,
+                       "peak_lane_ops_per_s": peak, "issue_frac": issue / peak,
                        "source": "profiles/sq_counters.json (SQ_INSTS_VALU per launch) / the live launch time"}
         if out["valu"]["issue_frac"] > out["frac"]:
             out["bound"] = "valu"
@@ -330,7 +334,8 @@ def fast_f64(sc, args, bitwise_value):
     g.close()
     out = {"value": sc.n_cells * args.steps / wall, "ms_per_step": wall / args.steps * 1e3,
            "over_bitwise_fp64": sc.n_cells * args.steps / wall / bitwise_value,
-           "roofline": roofline("f64", sc.n_cells, t1, t2, None, "stage_kernel<double, final> (fast build, per tile)",
+           "roofline": roofline("f64", sc.n_cells, t1, t2, _traffic("f64fast_final_stage_bytes_per_launch"),
+                                "stage_kernel<double, 32, 12, 512, final> (fast build, one workgroup per tile)",
                                 "f64fast_final")}
     n = 256
 
