@@ -272,10 +272,11 @@ def _traffic(key):
 def precision_form(sc, g64, args):
     """FP32 form on the headline workload: throughput, roofline, and depth
     accuracy against the FP64 grid `g64` (already advanced warmup + steps)
-    at FP64 step counts 100, 1000, 5000 and --acc-steps: the FP32 form and an
-    FP64 run on the bed perturbed by 1e-9 m (N(0, 1) * 1e-9, the workload's
-    own conditioning) are each run to the FP64 run's simulated time at every
-    checkpoint and compared with it there."""
+    at FP64 step counts 100, 1000, 5000 and --acc-steps: the FP32 form, the
+    fast FP64 kind and an FP64 run on the bed perturbed by 1e-9 m
+    (N(0, 1) * 1e-9, the workload's own conditioning) are each run to the
+    FP64 run's simulated time at every checkpoint and compared with it
+    there."""
     from hydroflux_amd.metrics import depth_metrics
 
     cells = sc.n_cells
@@ -295,6 +296,7 @@ def precision_form(sc, g64, args):
         marks = sorted(set(m for m in (100, 1000, 5000, args.acc_steps) if done <= m <= args.acc_steps))
         log(f"[bench] accuracy: FP64 to {marks} steps, FP32 and perturbed FP64 to the same times")
         g32 = build_grid(sc, "f32")
+        gf = build_grid(sc, "f64fast")
         rng = np.random.default_rng(12345)
         gp = build_grid(sc, "f64", bed=sc.bed + 1e-9 * rng.standard_normal(sc.bed.shape))
         curve = []
@@ -307,15 +309,21 @@ def precision_form(sc, g64, args):
             g32.run_until(t_ref)
             log(f"[bench]   FP32 at t = {t_ref:.0f} s")
             gp.run_until(t_ref)
+            gf.run_until(t_ref)
             row = {"fp64_steps": n_ref, "t_simulated_s": t_ref, "fp32_steps": g32.time()[1],
                    "fp32_vs_fp64": depth_metrics(g32.get_state()[..., 0], s64, sc.dx, sc.dx,
                                                  vol_test=g32.total_volume(), vol_ref=v64),
+                   "fast_fp64_steps": gf.time()[1],
+                   "fast_fp64_vs_fp64": depth_metrics(gf.get_state()[..., 0], s64, sc.dx, sc.dx,
+                                                      vol_test=gf.total_volume(), vol_ref=v64),
                    "fp64_bed_perturbed_1e-9m_vs_fp64": depth_metrics(gp.get_state()[..., 0], s64, sc.dx, sc.dx,
                                                                      vol_test=gp.total_volume(), vol_ref=v64)}
             curve.append(row)
             log(f"[bench]   {n_ref} steps, t = {t_ref:.0f} s: FP32 Linf {row['fp32_vs_fp64']['depth_Linf_m']:.3g} m, "
+                f"fast FP64 Linf {row['fast_fp64_vs_fp64']['depth_Linf_m']:.3g} m, "
                 f"perturbed FP64 Linf {row['fp64_bed_perturbed_1e-9m_vs_fp64']['depth_Linf_m']:.3g} m")
         g32.close()
+        gf.close()
         gp.close()
         out["accuracy"] = curve
     return out
@@ -323,7 +331,8 @@ def precision_form(sc, g64, args):
 
 def fast_f64(sc, args, bitwise_value):
     """The fast FP64 kind (HF2D_F64_FAST: contraction, reciprocal/Newton
-    division and square root) on the headline workload, and its drift from
+    division and square root, primitive-variable faces) on the headline
+    workload, and its drift from
     the bitwise kind on the 256 x 256 dam break (flat frictionless bed,
     walls, CFL 0.5) after 10, 100 and 500 steps."""
     from hydroflux_amd import Grid2D
