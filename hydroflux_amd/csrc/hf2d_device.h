@@ -47,9 +47,18 @@ template <class T> struct ShapeSmall {
 template <class T> struct ShapeMid {
   static constexpr bool on = false;
 };
+#ifndef HF_F64_MID_W
+#define HF_F64_MID_W 32
+#endif
+#ifndef HF_F64_MID_H
+#define HF_F64_MID_H 16
+#endif
+#ifndef HF_F64_MID_THREADS
+#define HF_F64_MID_THREADS 1024
+#endif
 template <> struct ShapeMid<double> {
   static constexpr bool on = true;
-  static constexpr int TW = 32, TH = 16, NT = 1024;
+  static constexpr int TW = HF_F64_MID_W, TH = HF_F64_MID_H, NT = HF_F64_MID_THREADS;
 };
 template <> struct ShapeSmall<double> {
   static constexpr bool on = true;
