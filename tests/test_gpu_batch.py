@@ -135,9 +135,10 @@ def test_batch_f32_alone_equals_in_mix_and_across_tile_configs():
 
 @pytest.mark.parametrize("lo,hi", [(200, 300), (470, 520)])
 def test_batch_f32_matches_single_grid_f32(lo, hi):
-    """The single grid takes 16 x 8 tiles below 160 tiles of 32 x 32 and
-    32 x 32 above; the batch runs 32 x 32. Tiles that divide the 64 x 64
-    bed-datum blocks give the FP32 form's bits whatever their shape."""
+    """The single grid takes 16 x 8 tiles below 160 tiles of 32 x 16 and
+    32 x 16 above; the batch runs its default configuration, 32 x 32. Tiles
+    that divide the 64 x 64 bed-datum blocks give the FP32 form's bits
+    whatever their shape."""
     _say(f"batch f32 vs single-grid f32 {lo}-{hi}")
     from hydroflux_amd import Grid2D
 
