@@ -247,6 +247,39 @@ def test_dualn_bitwise(n):
     assert np.array_equal(gv, ov) and np.array_equal(gt, ot, equal_nan=True)
 
 
+@pytest.mark.parametrize("n", [4, 8, 16])
+def test_dualn_component_pairs_bitwise(n):
+    """DualN<4> and DualN<8> on a grid of at least 512 tiles of 16 x 8 run
+    each stage as N / 2 launches of the DualN<2> kernel, one pair of tangent
+    components each; DualN<16> runs four groups of 4 on the DualN<4>
+    kernel's 8 x 8 tiles. 520 x 600 (2,475 tiles of 16 x 8): the fused
+    driver with rain and a point source, then the host loop, values and all
+    N tangents bit for bit with the oracle."""
+    _say(f"dualn<{n}> component pairs 520x600")
+    c = cases.bumpy_wet_dry(520, 600, 3)
+    c.manning = np.maximum(c.manning, 0.03)
+    rng = np.random.default_rng(100 + n)
+    kw = dict(manning_tangent=0.01 * rng.standard_normal((c.nr, c.nc, n)),
+              state_tangent=0.1 * rng.standard_normal((c.nr, c.nc, 3, n)))
+    g = c.load(_hf().Grid2D(c.nr, c.nc, c.dx, c.dy, kind=f"dual{n}"), **kw)
+    o = c.load(OracleSim(c.nr, c.nc, c.dx, c.dy, tangents=n), **kw)
+    for s in (g, o):
+        s.configure(cfl=0.4, max_dt=0.05, rain_rate=2e-4, sources=[(260, 300, 0.5)])
+        s.run_steps(6)
+    assert g.time() == o.time()
+    (gv, gt), (ov, ot) = g.get_state(), o.get_state()
+    assert np.array_equal(gv, ov) and np.array_equal(gt, ot, equal_nan=True)
+    for _ in range(3):
+        dg, do = g.cfl_time_step(0.4), o.cfl_time_step(0.4)
+        assert dg == do
+        g.ssprk2_step(dg)
+        o.ssprk2_step(do)
+        g.manning_friction_step(dg, 1e-6)
+        o.manning_friction_step(do, 1e-6)
+    (gv, gt), (ov, ot) = g.get_state(), o.get_state()
+    assert np.array_equal(gv, ov) and np.array_equal(gt, ot, equal_nan=True)
+
+
 @pytest.mark.parametrize("kind,nr,nc", [("f64", 520, 600), ("f64", 260, 300), ("f64", 150, 100)])
 def test_default_mid_and_small_grid_tiles_bitwise(kind, nr, nc):
     """A grid takes the default f64 stage tiles (32 x 24) when they number
