@@ -294,9 +294,10 @@ __global__ __launch_bounds__(NT) void stage_kernel(StageArgs<T> a) {
 // 32 x 12 with 512 threads, two workgroups per CU) measured 2.62 ms per step
 // one workgroup per tile and 2.95-3.16 ms persistent (capped at 128 VGPRs),
 // against 2.57 ms here. The fast FP64 kind takes the 32 x 12 per-tile form
-// (hf2d_fast.h), and the FP32 tile (32 x 16, 512 threads, two to three
-// workgroups per CU) runs per tile: its persistent form measured 1.13 /
-// 1.21 ms against 0.71 / 0.81 ms.
+// (hf2d_fast.h), and the FP32 tile (32 x 16, 512 threads, three workgroups
+// per CU) runs per tile: persistent at three workgroups per CU it measured
+// 1.01 / 1.13 ms per stage (0.66 / 0.74 ms with the load, slopes and faces
+// phases' index arithmetic recomputed per tile) against 0.53 / 0.62 ms.
 #ifndef HF_PERSIST
 #define HF_PERSIST 1
 #endif
