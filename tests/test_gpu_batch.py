@@ -127,8 +127,9 @@ def test_batch_f32_alone_equals_in_mix_and_across_tile_configs():
     alone = _run_batch("f32", specs[:1])[0]
     mix = _run_batch("f32", specs)[0]
     assert np.array_equal(alone[0], mix[0]) and alone[1:] == mix[1:]
-    # tile shapes whose sides divide the 64-cell datum block share the datum frame
-    for cfg in (1, 2, 3, 4):
+    # tile shapes whose sides divide the 64-cell datum block share the datum
+    # frame; the default is configuration 1 (32 x 16)
+    for cfg in (0, 2, 3, 4):
         other = _run_batch("f32", specs[:2], config=cfg)[0]
         assert np.array_equal(other[0], alone[0]), cfg
 
@@ -136,7 +137,7 @@ def test_batch_f32_alone_equals_in_mix_and_across_tile_configs():
 @pytest.mark.parametrize("lo,hi", [(200, 300), (470, 520)])
 def test_batch_f32_matches_single_grid_f32(lo, hi):
     """The single grid takes 16 x 8 tiles below 160 tiles of 32 x 16 and
-    32 x 16 above; the batch runs its default configuration, 32 x 32. Tiles
+    32 x 16 above; the batch runs its default configuration, 32 x 16. Tiles
     that divide the 64 x 64 bed-datum blocks give the FP32 form's bits
     whatever their shape."""
     _say(f"batch f32 vs single-grid f32 {lo}-{hi}")
