@@ -275,7 +275,7 @@ __global__ __launch_bounds__(NT) void stage_kernel(StageArgs<T> a) {
 // Stage kernels that run as a fixed grid of workgroups, each looping over
 // tiles w = blockIdx.x, blockIdx.x + gridDim.x, ... in tile_of_block's
 // order (a multiple of 8 workgroups keeps every workgroup on its XCD's
-// band): the FP64 tile (32 x 24, 768 threads) holds 150 KB of LDS, one
+// band): the FP64 tile (32 x 24, 768 threads) holds 143 KB of LDS, one
 // workgroup per CU, so the per-tile kernel leaves its CU idle through each
 // load phase and each workgroup's start. Here the next tile's global loads
 // are issued right after the current tile's are stored to LDS and overlap
@@ -293,7 +293,9 @@ __global__ __launch_bounds__(NT) void stage_kernel(StageArgs<T> a) {
 // The bitwise FP64 kernels keep this form. Tiles of half the LDS (24 x 16 or
 // 32 x 12 with 512 threads, two workgroups per CU) measured 2.62 ms per step
 // one workgroup per tile and 2.95-3.16 ms persistent (capped at 128 VGPRs),
-// against 2.57 ms here. The fast FP64 kind takes the 32 x 12 per-tile form
+// against 2.57 ms here; 1024 threads on 32 x 24 (four waves per SIMD at 128
+// VGPRs) spills in the final stage and measured 2.84-2.97 ms. The fast FP64
+// kind takes the 32 x 12 per-tile form
 // (hf2d_fast.h), and the FP32 tile (32 x 16, 512 threads, three workgroups
 // per CU) runs per tile: persistent at three workgroups per CU it measured
 // 1.01 / 1.13 ms per stage (0.66 / 0.74 ms with the load, slopes and faces
