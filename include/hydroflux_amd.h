@@ -61,9 +61,10 @@ enum hf2d_scalar_kind {
   HF2D_DUAL8 = 5,
   HF2D_DUAL16 = 6,
   /* FP64 with FMA contraction, reciprocal-and-Newton division (1/h shared
-   * by hu/h and hv/h) and Newton square root: the f64 path's arithmetic to
-   * about one ulp per operation, not bit for bit; no reference
-   * counterpart. Single grids only. */
+   * by hu/h and hv/h), Newton square root and interior faces evaluated in
+   * primitive variables: the f64 path's arithmetic to about one ulp per
+   * operation, not bit for bit; no reference counterpart. Single grids
+   * only. */
   HF2D_F64_FAST = 7
 };
 

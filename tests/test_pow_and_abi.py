@@ -122,6 +122,26 @@ def test_library_is_gfx950_code_object():
     assert b"gfx950" in blob
 
 
+def test_size_limits_rejected_before_any_device_call():
+    """Grid sides of 2^30 cells or more and batches of more than 16384 slots
+    are refused by argument checks that run before the library touches the
+    device (so this runs without a GPU); the handle stays null."""
+    from hydroflux_amd import _lib
+
+    L = _lib.lib()
+    h = ctypes.c_void_p()
+    side = 1 << 30
+    assert L.hf2d_create(side, 8, 1.0, 1.0, 0, ctypes.byref(h)) == 2  # HF2D_INVALID_CONFIG
+    assert L.hf2d_create(8, side, 1.0, 1.0, 0, ctypes.byref(h)) == 2
+    assert L.hf2d_create(8, 8, 0.0, 1.0, 0, ctypes.byref(h)) == 2
+    assert L.hf2d_create(8, 8, 1.0, 1.0, 99, ctypes.byref(h)) == 6  # unknown kind: HF2D_INVALID_ARGUMENT
+    assert h.value is None
+    for slots in (0, 16385):
+        assert L.hf2d_batch_create(2, 1 << 20, slots, ctypes.byref(h)) == 6
+    assert L.hf2d_batch_create(7, 1 << 20, 4, ctypes.byref(h)) == 6  # the fast kind is single-grid only
+    assert h.value is None
+
+
 def test_no_cpu_fallback_when_library_missing(monkeypatch, tmp_path):
     from hydroflux_amd import _lib
 
