@@ -57,10 +57,6 @@ struct hf2d_sim {
   int* tile_list = nullptr;
   unsigned long long* ws_part = nullptr;  // CFL slots of the final stage (WS_SLOTS x WS_STRIDE)
   int tw = 0, th = 0;  // stage tile shape: Shape<T>, ShapeMid<T> or ShapeSmall<T> (by grid size)
-  // DualN<4> / DualN<8> on a large grid: each stage runs as N / 2 launches
-  // of the DualN<2> kernel (its 16 x 8 tiles) instead of the DualN<4>
-  // kernel's 8 x 8 tiles
-  bool dn_pairs = false;
   int shape = 0;       // 0 Shape, 1 ShapeMid, 2 ShapeSmall
   int tiles_x = 0, tiles_y = 0;
   // workgroups of the persistent stage kernels (FP64 kinds, default tile):
@@ -201,30 +197,15 @@ template <class T> StageArgs<T> make_args(hf2d_sim* s, int in, int out, int un) 
 }
 
 template <class T> struct WideDualN {
-  static constexpr int n = 0, groups = 0;
+  static constexpr int n = 0;
 };
 template <int N> struct WideDualN<hf::dualn<N>> {
-  static constexpr int n = N, groups = N > 4 ? N / 4 : 0;
+  static constexpr int n = N;
 };
 
-// DualN<4> and DualN<8> grids with at least this many 16 x 8 tiles run
-// their stages as pairs of tangent components on the DualN<2> kernel, whose
-// 16 x 8 tile has 2x the cells of DualN<4>'s 8 x 8 and less halo per cell.
-// Measured cost per parameter against the f64 primal (m1_forward_scaling
-// construction), groups of 4 -> pairs: N = 4 at 256^2 / 512^2 / 1024^2 /
-// 2048^2 0.99 -> 0.85, 1.32 -> 1.23, 1.85 -> 1.75, 2.28 -> 2.20; N = 8
-// 1.09 -> 0.96, 1.47 -> 1.38, 2.13 -> 2.10, 2.52 -> 2.48. A pair reads 2 of
-// every N tangent doubles of a cell, so DualN<16> (2.39 -> 2.56 at 1024^2)
-// keeps groups of 4, and so does a small, launch-bound grid (64^2, N = 4:
-// 0.25 -> 0.54).
-#ifndef HF_DUALN_PAIR_TILES
-#define HF_DUALN_PAIR_TILES 512
-#endif
-
 // Component group c (tangents [G c, G c + G)) of a DualN<N> launch as a
-// DualN<G> launch (G = 4: the DualN<4> kernel's 8 x 8 tiles; G = 2: the
-// DualN<2> kernel's 16 x 8 tiles, which the grid then uses). Every group
-// computes the same values, bit for bit (a DualN value never depends on its
+// DualN<G> launch on the DualN<G> kernel's tiles (G = 4: 16 x 8). Every
+// group computes the same values, bit for bit (a DualN value never depends on its
 // tangents, and each tangent component follows the same rule whatever N
 // is); only the last group stores values and reduces the CFL wave speeds,
 // so the in-place final stage reads U^n values that no earlier group has
@@ -312,15 +293,6 @@ template <class T> int choose_tiles(hf2d_sim* s) {
     s->tw = HF2D_FAST_TILE_W;
     s->th = HF2D_FAST_TILE_H;
   }
-  if constexpr (WideDualN<T>::n == 4 || WideDualN<T>::n == 8) {
-    using P = Shape<hf::dualn<2>>;
-    if (tiles_of(s, P::TW, P::TH) >= HF_DUALN_PAIR_TILES) {
-      s->dn_pairs = true;
-      s->tw = P::TW;
-      s->th = P::TH;
-      return HF2D_OK;
-    }
-  }
   if (tiles_of(s, s->tw, s->th) >= SMALL_GRID_TILES) return HF2D_OK;
   if constexpr (ShapeMid<T>::on) {
     if (tiles_of(s, ShapeMid<T>::TW, ShapeMid<T>::TH) >= MID_GRID_TILES) {
@@ -375,19 +347,9 @@ template <class T> int launch_stage(hf2d_sim* s, StageArgs<T> a, int mode, int t
     HIPCHK(s, hipEventRecord(e0, s->stream));
   }
   constexpr int N = WideDualN<T>::n;
-  if constexpr (N >= 4) {
-    if (s->dn_pairs) {
-      for (int c = 0; c < N / 2; ++c) {
-        const int rc = launch_stage_kernel(s, group_args<2>(a, c), mode);
-        if (rc) return rc;
-      }
-    } else if constexpr (N > 4) {
-      for (int c = 0; c < N / 4; ++c) {
-        const int rc = launch_stage_kernel(s, group_args<4>(a, c), mode);
-        if (rc) return rc;
-      }
-    } else {
-      const int rc = launch_stage_kernel(s, a, mode);
+  if constexpr (N > 4) {
+    for (int c = 0; c < N / 4; ++c) {
+      const int rc = launch_stage_kernel(s, group_args<4>(a, c), mode);
       if (rc) return rc;
     }
   } else {

@@ -39,12 +39,14 @@ const CfgInfo CFG_F32[] = {{32, 32, 1024}, {32, 16, 512}, {16, 16, 256}, {32, 8,
 // 32 x 12 with 512 threads holds 79 KB of LDS: two workgroups per CU.
 const CfgInfo CFG_F64[] = {{32, 24, 768}, {32, 24, 1024}, {32, 16, 1024}, {32, 12, 512},
                            {16, 16, 256}, {32, 8, 256},   {64, 4, 256},   {128, 2, 256}};
-const CfgInfo CFG_DUAL[] = {{16, 8, 512}, {16, 16, 1024}, {16, 8, 256}};
+const CfgInfo CFG_DUAL[] = {{16, 8, 512}, {16, 16, 1024}, {16, 8, 256}, {24, 16, 1024}, {32, 12, 768}};
 // The configuration a new batch starts on: the fastest per stage over a
 // 256-scenario flood mix (98.7 M cells) on MI355X, FP32 32 x 16 x 512
 // (3.40 / 3.89 ms first / final stage vs 4.57 / 5.32 ms for 32 x 32) and
-// FP64 32 x 12 x 512 (7.43 / 9.07 ms vs 7.59 / 9.34 ms for 32 x 24 x 768).
-constexpr int CFG_F32_DEFAULT = 1, CFG_F64_DEFAULT = 3, CFG_DUAL_DEFAULT = 0;
+// FP64 32 x 12 x 512 (7.43 / 9.07 ms vs 7.59 / 9.34 ms for 32 x 24 x 768);
+// Dual 24 x 16 x 1024 on the mix's first 64 scenarios (29.2 M cells):
+// 11.7 ms per step against 14.2 ms for 16 x 8 x 512.
+constexpr int CFG_F32_DEFAULT = 1, CFG_F64_DEFAULT = 3, CFG_DUAL_DEFAULT = 3;
 
 template <class T> struct Cfgs;
 template <> struct Cfgs<float> {
@@ -57,7 +59,7 @@ template <> struct Cfgs<double> {
 };
 template <> struct Cfgs<dual> {
   static const CfgInfo* list() { return CFG_DUAL; }
-  static constexpr int n = 3;
+  static constexpr int n = 5;
 };
 
 template <class T, int TW, int TH, int NT>
@@ -100,7 +102,9 @@ template <> int launch_cfg<dual>(int ci, hipStream_t st, const StageArgs<dual>* 
   switch (ci) {
     case 0: return launch_pair<dual, 16, 8, 512>(st, d1, d2, w, n);
     case 1: return launch_pair<dual, 16, 16, 1024>(st, d1, d2, w, n);
-    default: return launch_pair<dual, 16, 8, 256>(st, d1, d2, w, n);
+    case 2: return launch_pair<dual, 16, 8, 256>(st, d1, d2, w, n);
+    case 3: return launch_pair<dual, 24, 16, 1024>(st, d1, d2, w, n);
+    default: return launch_pair<dual, 32, 12, 768>(st, d1, d2, w, n);
   }
 }
 

@@ -64,6 +64,10 @@ template <> struct ShapeSmall<double> {
   static constexpr bool on = true;
   static constexpr int TW = 16, TH = 8, NT = 512;
 };
+template <> struct ShapeSmall<dual> {
+  static constexpr bool on = true;
+  static constexpr int TW = 16, TH = 8, NT = 512;
+};
 template <> struct ShapeSmall<float> {
   static constexpr bool on = true;
   static constexpr int TW = 16, TH = 8, NT = 512;
@@ -94,23 +98,36 @@ __device__ inline unsigned tile_of_block(unsigned b, unsigned nb) {
 template <> struct Shape<double> {
   static constexpr int TW = HF_F64_TILE_W, TH = HF_F64_TILE_H, NT = HF_F64_THREADS;
 };
+// Dual: 24 x 16 tiles with 1024 threads, 152 KB of LDS, one workgroup and
+// four waves per SIMD per CU; 2.55 faces computed per cell against 3.03 on
+// 16 x 8 tiles. Measured on the flood workload (G cell-updates/s, 1024^2 /
+// 4096^2): 16 x 8 x 512 (two workgroups per CU) 1.97 / 2.13, 32 x 12 x 768
+// 2.38 / 2.54, 32 x 12 x 1024 2.43 / 2.57, 24 x 16 x 768 2.40 / 2.54,
+// 24 x 16 x 1024 2.45 / 2.59; the persistent form on 32 x 12 2.14-2.22 /
+// 2.38-2.52. Grids of fewer than SMALL_GRID_TILES such tiles take 16 x 8.
 #ifndef HF_DUAL_TILE_W
-#define HF_DUAL_TILE_W 16
+#define HF_DUAL_TILE_W 24
 #endif
 #ifndef HF_DUAL_TILE_H
-#define HF_DUAL_TILE_H 8
+#define HF_DUAL_TILE_H 16
 #endif
 #ifndef HF_DUAL_THREADS
-#define HF_DUAL_THREADS 512
+#define HF_DUAL_THREADS 1024
 #endif
 template <> struct Shape<dual> {
   static constexpr int TW = HF_DUAL_TILE_W, TH = HF_DUAL_TILE_H, NT = HF_DUAL_THREADS;
 };
 #ifndef HF_DUAL2_THREADS
-#define HF_DUAL2_THREADS 512
+#define HF_DUAL2_THREADS 768
 #endif
 #ifndef HF_DUAL4_THREADS
 #define HF_DUAL4_THREADS 512
+#endif
+#ifndef HF_DUAL4_TILE_W
+#define HF_DUAL4_TILE_W 16
+#endif
+#ifndef HF_DUAL4_TILE_H
+#define HF_DUAL4_TILE_H 8
 #endif
 // DualN<N>: (N + 1) doubles per LDS value; tiles sized to stay within one
 // workgroup's LDS. For N > 4 the stage kernels run as N / 4 launches of the
@@ -119,12 +136,43 @@ template <> struct Shape<dual> {
 // cells loaded per cell updated), so its tiles are DualN<4>'s.
 template <int N> struct Shape<hf::dualn<N>> {
   static_assert(N > 4 && N % 4 == 0, "DualN<N> wider than 4 runs in groups of 4 components");
-  static constexpr int TW = 8, TH = 8, NT = HF_DUAL4_THREADS;
+  static constexpr int TW = HF_DUAL4_TILE_W, TH = HF_DUAL4_TILE_H, NT = HF_DUAL4_THREADS;
 };
+// DualN<2>: 24 x 8 tiles with 768 threads (134 KB of LDS, one workgroup and
+// three waves per SIMD per CU), 2.9 faces per cell against 3.03 on 16 x 8.
+// Measured at 2048^2 (flood workload, G cell-updates/s): 16 x 8 x 512 1.12,
+// 16 x 12 x 768 1.42, 24 x 8 x 768 1.43. Grids of fewer than
+// SMALL_GRID_TILES such tiles take 16 x 8.
+// DualN<4> (and the groups of 4 of wider DualN): 16 x 8 tiles with 512
+// threads (159 KB of LDS; 246-256 VGPRs, so one workgroup of two waves per
+// SIMD per CU whatever the tile). Against 8 x 8 tiles, which load 3.1
+// cells per cell updated and keep one wave busy in the update phase:
+// DualN<16> at 1024^2 9.20 -> 6.12 ms per step, DualN<4> at 256^2 0.177 ->
+// 0.112 ms, 12 x 8 in between. On these tiles groups of 4 also beat pairs
+// of components on the DualN<2> kernel (2048^2: DualN<4> 5.84 -> 5.00 ms,
+// DualN<8> 12.5 -> 10.3 ms).
+#ifndef HF_DUAL2_TILE_W
+#define HF_DUAL2_TILE_W 24
+#endif
+#ifndef HF_DUAL2_TILE_H
+#define HF_DUAL2_TILE_H 8
+#endif
 template <> struct Shape<hf::dualn<2>> {
-  static constexpr int TW = 16, TH = 8, NT = HF_DUAL2_THREADS;
+  static constexpr int TW = HF_DUAL2_TILE_W, TH = HF_DUAL2_TILE_H, NT = HF_DUAL2_THREADS;
+};
+template <> struct ShapeSmall<hf::dualn<2>> {
+  static constexpr bool on = true;
+  static constexpr int TW = 16, TH = 8, NT = 512;
 };
 template <> struct Shape<hf::dualn<4>> {
+  static constexpr int TW = HF_DUAL4_TILE_W, TH = HF_DUAL4_TILE_H, NT = HF_DUAL4_THREADS;
+};
+// Grids of fewer than SMALL_GRID_TILES 16 x 8 tiles: DualN<4> and the
+// groups of 4 of wider DualN take 8 x 8 (64^2, one DualN<P> pass against
+// the f64 primal, cost per parameter P = 4 / 8 / 16: 0.27 / 0.37 / 0.43 on
+// 8 x 8, 0.29 / 0.40 / 0.46 on 16 x 8).
+template <int N> struct ShapeSmall<hf::dualn<N>> {
+  static constexpr bool on = N >= 4;
   static constexpr int TW = 8, TH = 8, NT = HF_DUAL4_THREADS;
 };
 // FP32 tiles lie inside one 64 x 64 bed-datum block (shapes that divide

@@ -305,7 +305,7 @@ int hf2d_batch_download_done(hf2d_batch* batch, int32_t slot, double* state_aos,
 /* Tile-shape / threads-per-workgroup configurations and the autotuner that
  * times each on the resident mix (states are restored afterwards). A new
  * batch starts on the shape measured fastest on a large flood mix: FP32
- * 32 x 16 x 512, FP64 32 x 12 x 512, Dual 16 x 8 x 512. */
+ * 32 x 16 x 512, FP64 32 x 12 x 512, Dual 24 x 16 x 1024. */
 int hf2d_batch_config_count(const hf2d_batch* batch, int32_t* count);
 int hf2d_batch_config_info(const hf2d_batch* batch, int32_t index, int32_t* tile_w, int32_t* tile_h,
                            int32_t* threads);

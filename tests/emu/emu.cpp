@@ -325,7 +325,7 @@ template <class T> struct Dispatch {
   template <class F> static void with_shape(int tw, int th, F f);
 };
 
-#define HF_EMU_SHAPES(X) X(64, 24) X(32, 24) X(32, 16) X(16, 16) X(8, 4) X(4, 8) X(16, 8) X(24, 16)
+#define HF_EMU_SHAPES(X) X(64, 24) X(32, 24) X(32, 16) X(16, 16) X(8, 4) X(4, 8) X(16, 8) X(24, 16) X(24, 8)
 
 template <class T, int TW, int TH> struct Ops {
   static void ssprk2(Emu& e, double dt) { host_dt_step<T, TW, TH>(e, dt, true); }

@@ -247,15 +247,14 @@ def test_dualn_bitwise(n):
     assert np.array_equal(gv, ov) and np.array_equal(gt, ot, equal_nan=True)
 
 
-@pytest.mark.parametrize("n", [4, 8, 16])
-def test_dualn_component_pairs_bitwise(n):
-    """DualN<4> and DualN<8> on a grid of at least 512 tiles of 16 x 8 run
-    each stage as N / 2 launches of the DualN<2> kernel, one pair of tangent
-    components each; DualN<16> runs four groups of 4 on the DualN<4>
-    kernel's 8 x 8 tiles. 520 x 600 (2,475 tiles of 16 x 8): the fused
-    driver with rain and a point source, then the host loop, values and all
-    N tangents bit for bit with the oracle."""
-    _say(f"dualn<{n}> component pairs 520x600")
+@pytest.mark.parametrize("n", [2, 4, 8, 16])
+def test_dualn_large_tiles_and_groups_bitwise(n):
+    """DualN<2> on its 24 x 8 tiles, DualN<4> on its 16 x 8 tiles, DualN<8>
+    and DualN<16> as 2 and 4 launches of the DualN<4> kernel per stage, one
+    group of 4 tangent components each. 520 x 600 (1,625 tiles of 24 x 8,
+    2,475 of 16 x 8): the fused driver with rain and a point source, then
+    the host loop, values and all N tangents bit for bit with the oracle."""
+    _say(f"dualn<{n}> default tiles 520x600")
     c = cases.bumpy_wet_dry(520, 600, 3)
     c.manning = np.maximum(c.manning, 0.03)
     rng = np.random.default_rng(100 + n)
@@ -280,17 +279,19 @@ def test_dualn_component_pairs_bitwise(n):
     assert np.array_equal(gv, ov) and np.array_equal(gt, ot, equal_nan=True)
 
 
-@pytest.mark.parametrize("kind,nr,nc", [("f64", 520, 600), ("f64", 260, 300), ("f64", 150, 100)])
+@pytest.mark.parametrize("kind,nr,nc", [("f64", 520, 600), ("f64", 260, 300), ("f64", 150, 100), ("dual", 520, 600),
+                                         ("dual", 150, 100)])
 def test_default_mid_and_small_grid_tiles_bitwise(kind, nr, nc):
     """A grid takes the default f64 stage tiles (32 x 24) when they number
     at least 160, else 32 x 16 tiles when those number at least 128, else
     16 x 8: 520 x 600 (418 default tiles), 260 x 300 (110 default, 170 of
     32 x 16) and 150 x 100 (40 of 32 x 16) run the three kernels, each bit
     for bit with the oracle through the fused driver with friction, rain
-    and point sources and the host loop."""
+    and point sources and the host loop. Dual takes 24 x 16 tiles from 160
+    of them (520 x 600: 726) and 16 x 8 below (150 x 100: 70 of 24 x 16)."""
     _say(f"large-tile kernels {kind} {nr}x{nc}")
     c = cases.bumpy_wet_dry(nr, nc, 3)
-    kw = {}
+    kw = {"manning_tangent": np.where(c.manning > 0.04, 1.0, 0.0)} if kind == "dual" else {}
     g = c.load(_hf().Grid2D(c.nr, c.nc, c.dx, c.dy, kind=kind), **kw)
     o = c.load(OracleSim(c.nr, c.nc, c.dx, c.dy, dual=kind == "dual"), **kw)
     for s in (g, o):

@@ -22,7 +22,7 @@ def _mode():
     set_pow_mode(0)
 
 
-TILES = [(64, 24), (32, 24), (32, 16), (8, 4), (4, 8)]
+TILES = [(64, 24), (32, 24), (32, 16), (24, 16), (24, 8), (8, 4), (4, 8)]
 CASES = [
     (cases.stoker(100), None),
     (cases.thacker(40), None),
@@ -64,7 +64,7 @@ def test_fused_driver_bitwise(tile, integrator):
     assert np.array_equal(o.get_state(), e.get_state())
 
 
-@pytest.mark.parametrize("tile", [(16, 16), (8, 4)], ids=lambda t: f"{t[0]}x{t[1]}")
+@pytest.mark.parametrize("tile", [(16, 16), (8, 4), (24, 16), (16, 8)], ids=lambda t: f"{t[0]}x{t[1]}")
 def test_dual_driver_bitwise(tile):
     c = cases.bumpy_wet_dry(19, 27, 9)
     c.manning = np.maximum(c.manning, 0.03)
@@ -99,7 +99,7 @@ def test_driver_errors_match_oracle():
         assert s.time()[1] == 3
 
 
-@pytest.mark.parametrize("tile", [(8, 4), (4, 8), (16, 8), (32, 16), (32, 24)], ids=lambda t: f"{t[0]}x{t[1]}")
+@pytest.mark.parametrize("tile", [(8, 4), (4, 8), (16, 8), (32, 16), (32, 24), (24, 8)], ids=lambda t: f"{t[0]}x{t[1]}")
 @pytest.mark.parametrize("west", [cases.DISCHARGE, cases.DEPTH], ids=["discharge", "depth"])
 def test_active_tile_list_bitwise(tile, west):
     """The compacted active-tile list leaves settled dry tiles out; results
