@@ -111,6 +111,60 @@ def test_draining_cell_regression_and_mass():
     assert abs(out[..., 0].sum() - m0) < 1e-12
 
 
+H_DRY = 1.0e-6  # update.rs: the depth below which a cell is dry
+
+
+def _ulps(x):
+    return float(np.spacing(np.float64(x)))
+
+
+@pytest.mark.parametrize("dt", [0.2, 0.5, 1.0])
+def test_alpha_rescaling_closed_form_on_a_draining_chain(dt):
+    """Liang-Marche outflow limiter (update.rs:953-1053) where it binds.
+
+    One row between walls: A (h 0.05, u 5) upstream of B (h 0.02, u 5), then
+    dry cells, with a dt at which both would empty. alpha = (h - H_DRY) / out
+    counts outflows only and scales each face by its upstream cell's alpha,
+    so exactly the available depth of each wet cell moves one cell
+    downstream: A keeps H_DRY, B receives all of A's (ending at h_A), the
+    first dry cell receives all of B's. Rounding aside, nothing else is
+    free: using the receiving cell's alpha, counting inflow in `out`, or
+    dropping H_DRY from the available depth each moves these values by far
+    more than the few ulps allowed."""
+    s = _sim(1, 5)
+    st = np.zeros((1, 5, 3))
+    st[0, 0] = (0.05, 0.25, 0.0)
+    st[0, 1] = (0.02, 0.10, 0.0)
+    s.set_state(st)
+    s.set_boundaries([1, 1, 1, 1])
+    s.forward_euler_step(dt)
+    h = s.get_state()[0, :, 0]
+    tol = 4 * _ulps(0.05)
+    assert abs(h[0] - H_DRY) <= tol, h
+    assert abs(h[1] - 0.05) <= tol, h
+    assert abs(h[2] - (0.02 - H_DRY)) <= tol, h
+    assert h[3] == 0.0 and h[4] == 0.0
+    assert abs(h.sum() - 0.07) <= tol
+
+
+@pytest.mark.parametrize("angle", [0.0, 0.7, 1.9, 3.3, 5.1])
+def test_alpha_rescaling_isolated_cell_empties_to_h_dry(angle):
+    """An isolated wet cell among dry ones has only outflows: where the
+    limiter binds (update.rs:953-982) its new depth is h - alpha*out =
+    H_DRY to rounding, and the water it loses is what its neighbours gain."""
+    h0, speed = 0.05, 6.0
+    s = _sim(3, 3)
+    st = np.zeros((3, 3, 3))
+    st[1, 1] = (h0, h0 * speed * math.cos(angle), h0 * speed * math.sin(angle))
+    s.set_state(st)
+    s.set_boundaries([1, 1, 1, 1])
+    s.forward_euler_step(0.5)
+    out = s.get_state()[..., 0]
+    assert abs(out[1, 1] - H_DRY) <= 4 * _ulps(h0), out[1, 1]
+    assert abs(out.sum() - h0) <= 4 * _ulps(h0)
+    assert np.all(out >= 0.0)
+
+
 def test_wetting_front_conserves_mass_closed_box():
     # update.rs:1689-1717
     n = 30
