@@ -428,31 +428,69 @@ def uk_ea(args):
     Test 4 (400 x 200, 5 m) and Test 8A (Glasgow, 199 x 481, 2 m). The
     reference's recorded CPU wall times of the same runs, on its authors'
     machine (papers/01_review/ROADMAP_REVISION_EMS.md:211-229), are given for
-    context only."""
+    context only. Each test also runs in the FP32 precision form, and in FP64
+    with the bed perturbed by up to 1e-9 m (the run's own conditioning): both
+    are compared with the FP64 run (final depth field and the control-point
+    series)."""
     from hydroflux_amd import uk_ea as U
+    from hydroflux_amd.ascii_grid import read_ascii_grid
+    from hydroflux_amd.metrics import depth_metrics
 
     g4 = os.path.join(ROOT, "tests", "golden", "uk_ea", "test4")
     g8 = os.path.join(ROOT, "tests", "golden", "uk_ea", "test8a")
+    dem4 = os.path.join(g4, "ea4-5m.dem.gz")
+    dem8, man8 = os.path.join(g8, "ea8-2m.dem.gz"), os.path.join(g8, "ea8-2m.n.gz")
+
+    def make4(kind, pert):
+        g, _ = U.ea_test4_grid(dem4, kind=kind)
+        if pert:
+            bed, _ = read_ascii_grid(dem4)
+            g.set_mesh(bed + pert * np.random.default_rng(5).uniform(-1, 1, bed.shape), U.MANNING_N)
+        return g
+
+    def make8(kind, pert):
+        g, _ = U.ea_test8a_grid(dem8, man8, kind=kind)
+        if pert:
+            bed, _ = read_ascii_grid(dem8)
+            man, _ = read_ascii_grid(man8)
+            g.set_mesh(bed + pert * np.random.default_rng(5).uniform(-1, 1, bed.shape), man)
+        return g
+
     out = {}
-    for name, make, ref_s in (
-        ("test4", lambda: U.ea_test4_grid(os.path.join(g4, "ea4-5m.dem.gz")), 1136.0),
-        ("test8a", lambda: U.ea_test8a_grid(os.path.join(g8, "ea8-2m.dem.gz"), os.path.join(g8, "ea8-2m.n.gz")),
-         5289.0),
-    ):
-        g, _ = make()
-        cells = g.n_rows * g.n_cols
-        g.synchronize()
-        t0 = time.perf_counter()
-        t_end = args.ukea_time if args.ukea_time > 0 else U.SIM_TIME
-        g.run_until(t_end)
-        g.synchronize()
-        wall = time.perf_counter() - t0
-        steps = g.time()[1]
-        out[name] = {"grid": [g.n_rows, g.n_cols], "simulated_s": t_end, "steps": steps, "wall_s": wall,
-                     "cell_updates_per_s": cells * steps / wall, "us_per_step": wall / steps * 1e6,
-                     "reference_cpu_wall_s": ref_s}
-        g.close()
-        log(f"[bench] UK EA {name}: {steps} steps in {wall:.2f} s ({wall / steps * 1e6:.1f} us/step)")
+    for name, make, ref_s, dx in (("test4", make4, 1136.0, 5.0), ("test8a", make8, 5289.0, 2.0)):
+        runs = {}
+        for kind, pert in (("f64", 0.0), ("f32", 0.0), ("f64", 1e-9)):
+            g = make(kind, pert)
+            cells = g.n_rows * g.n_cols
+            g.synchronize()
+            t0 = time.perf_counter()
+            t_end = args.ukea_time if args.ukea_time > 0 else U.SIM_TIME
+            g.run_until(t_end)
+            g.synchronize()
+            wall = time.perf_counter() - t0
+            steps = g.time()[1]
+            times, depths = g.read_gauges()
+            runs[(kind, pert)] = (g.get_state()[..., 0].copy(), g.total_volume(), np.asarray(times),
+                                  np.asarray(depths), steps, wall)
+            g.close()
+            log(f"[bench] UK EA {name} {kind}{' bed+1e-9' if pert else ''}: {steps} steps in {wall:.2f} s "
+                f"({wall / steps * 1e6:.1f} us/step)")
+        h64, v64, t64, d64, steps, wall = runs[("f64", 0.0)]
+        r = {"grid": list(h64.shape), "simulated_s": t_end, "steps": steps, "wall_s": wall,
+             "cell_updates_per_s": h64.size * steps / wall, "us_per_step": wall / steps * 1e6,
+             "reference_cpu_wall_s": ref_s}
+        for key, label in ((("f32", 0.0), "fp32_form_vs_fp64"), (("f64", 1e-9), "fp64_bed_perturbed_1e-9m_vs_fp64")):
+            h, v, tt, dd, st, wl = runs[key]
+            m = depth_metrics(h, h64, dx, dx, vol_test=v, vol_ref=v64)
+            m["steps"] = st
+            m["control_point_max_abs_diff_m"] = max(
+                float(np.max(np.abs(np.interp(t64, tt, dd[:, p]) - d64[:, p]))) for p in range(d64.shape[1]))
+            m["control_point_peak_max_abs_diff_m"] = max(
+                abs(float(dd[:, p].max() - d64[:, p].max())) for p in range(d64.shape[1]))
+            if key[0] == "f32":
+                m["us_per_step"] = wl / st * 1e6
+            r[label] = m
+        out[name] = r
     return out
 
 

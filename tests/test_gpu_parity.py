@@ -427,6 +427,46 @@ def test_dual_inverse_manning_trace_on_gpu(golden_dir):
         n = min(max(n - err / dc if abs(dc) > 1e-12 else n, 1e-4), 1.0)
 
 
+@pytest.mark.parametrize("kind", ["f64", "f64fast"])
+def test_outflow_limiter_closed_form_on_gpu(kind):
+    """The Liang-Marche limiter where it binds (update.rs:953-1053) on the
+    device: on a one-row chain of two draining cells between walls each
+    cell's available depth (h - 1e-6) moves exactly one cell downstream, and
+    an isolated draining cell ends at 1e-6 (to 4 ulp of the depths; the
+    fast kind's reciprocal divisions to 1e-15 m); the bitwise kind also
+    equals the oracle bit for bit."""
+    _say(f"outflow limiter closed form {kind}")
+    h_dry = 1.0e-6
+    tol = 4 * float(np.spacing(0.05)) if kind == "f64" else 1e-15
+    st = np.zeros((1, 5, 3))
+    st[0, 0] = (0.05, 0.25, 0.0)
+    st[0, 1] = (0.02, 0.10, 0.0)
+    iso = np.zeros((3, 3, 3))
+    iso[1, 1] = (0.05, 0.05 * 6.0 * math.cos(1.9), 0.05 * 6.0 * math.sin(1.9))
+    for state, dt in ((st, 0.5), (iso, 0.5)):
+        nr, nc = state.shape[:2]
+        outs = []
+        sims = [_hf().Grid2D(nr, nc, 1.0, 1.0, kind=kind)]
+        if kind == "f64":
+            set_pow_mode(1)
+            sims.append(OracleSim(nr, nc, 1.0, 1.0))
+        for s in sims:
+            s.set_mesh(np.zeros((nr, nc)), 0.0)
+            s.set_state(state)
+            s.set_boundaries([1, 1, 1, 1])
+            s.forward_euler_step(dt)
+            outs.append(s.get_state())
+        h = outs[0][..., 0]
+        if nr == 1:
+            assert abs(h[0, 0] - h_dry) <= tol and abs(h[0, 1] - 0.05) <= tol, h
+            assert abs(h[0, 2] - (0.02 - h_dry)) <= tol and h[0, 3] == 0.0 and h[0, 4] == 0.0, h
+        else:
+            assert abs(h[1, 1] - h_dry) <= tol and np.all(h >= 0.0), h
+        assert abs(h.sum() - state[..., 0].sum()) <= tol
+        if len(outs) == 2:
+            assert np.array_equal(outs[0], outs[1])
+
+
 def test_lake_at_rest_and_mass_on_gpu():
     _say("lake at rest + mass")
     n = 64
