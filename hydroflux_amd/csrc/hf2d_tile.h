@@ -270,15 +270,15 @@ HF_HD float add_rounded_stochastically(float h, float dh, uint32_t key) {
   const float s = h + dh;
   const float bp = s - h;
   const float e = (h - (s - bp)) + (dh - bp);  // exact: (h + dh) - s (TwoSum)
-  if (e == 0.0f) return s;
+  if (e == 0.0f || !(s > 0.0f)) return s;
   uint32_t b;
   __builtin_memcpy(&b, &s, sizeof b);
   b = e > 0.0f ? b + 1u : b - 1u;  // the other neighbour of h + dh (s > 0)
   float nb;
   __builtin_memcpy(&nb, &b, sizeof nb);
-  const float p = e / (nb - s);                          // in (0, 1/2]: nb - s is exact
+  // take nb with probability |e| / |nb - s| (nb - s is exact), without a division
   const float u = (float)(hash32(key) >> 8) * 0x1p-24f;  // [0, 1)
-  return u < p ? nb : s;
+  return u * __builtin_fabsf(nb - s) < __builtin_fabsf(e) ? nb : s;
 }
 
 template <class T> HF_HD T minmod(T a, T b) {
@@ -1383,6 +1383,12 @@ template <class T, int TW, int TH> struct Stage {
     const R dt_dx = R(dt / a.dx), dt_dy = R(dt / a.dy), rdt = R(dt);
     ForcingNow fnow{};
     if (MODE == MODE_FINAL_RK2) fnow = forcing_now(a, gi0, gj0, TH, TW);
+    // FP32 form: the depth update h - dh and the RK combine round
+    // stochastically too (keyed by step, cell and which addition), for the
+    // same reason as the rain: in slowly varying flow a cell's small
+    // increment rounds the same way step after step.
+    uint32_t sr_step = 0u;
+    if constexpr (std::is_same<T, float>::value) sr_step = a.ctrl ? (uint32_t)a.ctrl->steps * 0x9E3779B9u : 0u;
     for (int k = tid; k < TW * TH; k += nt) {
       const int li = k / TW, lj = k % TW;
       const int gi = gi0 + li, gj = gj0 + lj;
@@ -1409,6 +1415,9 @@ template <class T, int TW, int TH> struct Stage {
         // right face .minus, left face .plus, bottom face .minus, top face .plus
         T dh = (rm - lm) * dt_dx + (bm - tm) * dt_dy;
         T new_h = h_old - dh;
+        if constexpr (std::is_same<T, float>::value)
+          new_h = add_rounded_stochastically(h_old, -dh, sr_step ^ ((uint32_t)idx * 0x85EBCA6Bu) ^
+                                                             (MODE == MODE_FINAL_RK2 ? 0x68E31DA4u : 0xB5297A4Du));
         T dhu = (ra - lb) * dt_dx + (bt - tt) * dt_dy;
         T dhv = (rt - lt) * dt_dx + (ba - tb) * dt_dy;
 
@@ -1485,7 +1494,10 @@ template <class T, int TW, int TH> struct Stage {
       }
       if (MODE == MODE_FINAL_RK2) {
         T ph = pk ? pk->ph : a.nh.ld(idx), phu = pk ? pk->phu : a.nhu.ld(idx), phv = pk ? pk->phv : a.nhv.ld(idx);
-        nh = (ph + nh) * R(0.5);
+        if constexpr (std::is_same<T, float>::value)
+          nh = add_rounded_stochastically(ph, nh, sr_step ^ ((uint32_t)idx * 0x85EBCA6Bu) ^ 0x1B56C4E9u) * 0.5f;
+        else
+          nh = (ph + nh) * R(0.5);
         nhu = (phu + nhu) * R(0.5);
         nhv = (phv + nhv) * R(0.5);
         if (val(nh) <= R(H_VEL)) {

@@ -98,19 +98,22 @@ def test_f32_form_on_uk_ea_test4_within_1mm():
     """UK EA Test 4 (flat 400 x 200 grid at 5 m, inflow hydrograph, 18000 s,
     53,610 steps): a well-conditioned run, whose FP64 depths move by 2e-9 m
     when the bed is perturbed by 1e-9 m. There the FP32 form meets the 1 mm
-    depth L-infinity bound against FP64 over the whole field. Measured on
-    MI355X: L-inf 2.9e-4 m, L1 1.6e-5 m, peak 7e-6 m, IoU 1.0, control-point
-    series within 1.0e-5 m, volume 1.2e-4 relative."""
+    depth L-infinity bound against FP64 over the whole field, with three
+    orders of magnitude to spare. Measured on MI355X: L-inf 5.8e-7 m, L1
+    4.8e-8 m, peak 5.6e-8 m, IoU 1.0, control-point series within 2.1e-7 m,
+    volume 3.4e-7 relative. (With round-to-nearest depth updates instead of
+    stochastic rounding: L-inf 2.9e-4 m, L1 1.6e-5 m, volume 1.2e-4; the
+    bounds below fail on that build.)"""
     print("[gpu-case] FP32 form vs FP64 on UK EA Test 4", flush=True)
     ref = _ukea_run("test4", "f64")
     m = _compare(_ukea_run("test4", "f32"), ref, 5.0)
     print(f"[gpu-case] {m}", flush=True)
-    assert m["depth_Linf_m"] <= 1e-3
-    assert m["depth_L1_m"] <= 1e-4
-    assert m["peak_depth_err_m"] <= 1e-4
+    assert m["depth_Linf_m"] <= 1e-5
+    assert m["depth_L1_m"] <= 1e-6
+    assert m["peak_depth_err_m"] <= 1e-6
     assert m["extent_iou_1cm"] >= 0.999
-    assert m["gauge_max"] <= 1e-4
-    assert m["mass_balance_rel_err"] <= 1e-3
+    assert m["gauge_max"] <= 2e-6
+    assert abs(m["mass_balance_rel_err"]) <= 1e-5
 
 
 def test_f32_form_on_uk_ea_test8a_within_fp64_conditioning():
@@ -118,8 +121,8 @@ def test_f32_form_on_uk_ea_test8a_within_fp64_conditioning():
     ill-conditioned, FP64 itself moves by 8 mm (L-inf) when the bed is
     perturbed by 1e-9 m. The FP32 form's L-inf stays within a small multiple
     of that, and its aggregate metrics stay tight. Measured on MI355X: FP32
-    L-inf 3.0e-2 m against the perturbed FP64's 8.0e-3 m; L1 3.9e-5 m, peak
-    2.6e-4 m, IoU 0.996, control-point peaks within 7.1e-4 m."""
+    L-inf 2.0e-2 m against the perturbed FP64's 8.0e-3 m; L1 8.9e-5 m, peak
+    6.8e-4 m, IoU 0.992, control-point peaks within 3.5e-3 m."""
     print("[gpu-case] FP32 form vs FP64 on UK EA Test 8A", flush=True)
     ref = _ukea_run("test8a", "f64")
     m = _compare(_ukea_run("test8a", "f32"), ref, 2.0)
@@ -129,8 +132,8 @@ def test_f32_form_on_uk_ea_test8a_within_fp64_conditioning():
     assert m["depth_Linf_m"] <= 10 * p["depth_Linf_m"]
     assert m["depth_L1_m"] <= 2e-4
     assert m["peak_depth_err_m"] <= 2e-3
-    assert m["extent_iou_1cm"] >= 0.99
-    assert m["gauge_peak"] <= 5e-3
+    assert m["extent_iou_1cm"] >= 0.985
+    assert m["gauge_peak"] <= 1e-2
     assert m["mass_balance_rel_err"] <= 2e-3
 
 
@@ -140,8 +143,9 @@ def test_f32_form_rain_keeps_its_mass():
     step is a few FP32 ulps of a deep cell; rounded to nearest, every cell
     of a binade rounds the same way on every step, which lost 7.2e-5 of the
     volume (4.5 % of the rain) here after 5,000 steps. The FP32 form rounds
-    the rain addition stochastically (exact in expectation): measured on
-    MI355X, -1.3e-6. FP64 on the same run is the exact budget."""
+    the rain addition stochastically (exact in expectation), and its depth
+    updates too: measured on MI355X, -6.2e-8 (-1.3e-6 with only the rain
+    rounded so). FP64 on the same run is the exact budget."""
     print("[gpu-case] FP32 form rain mass on a closed fBm box", flush=True)
     from hydroflux_amd import Grid2D
 
@@ -164,4 +168,4 @@ def test_f32_form_rain_keeps_its_mass():
         rel[kind] = (v - (v0 + rate * t * n * n * dx * dx)) / v
     print(f"[gpu-case] volume against the rain budget: {rel}", flush=True)
     assert abs(rel["f64"]) <= 1e-12
-    assert abs(rel["f32"]) <= 5e-6
+    assert abs(rel["f32"]) <= 1e-6
