@@ -520,4 +520,10 @@ void emu_time(void* p, double* t, int64_t* steps) {
   *t = e->ctrl.t;
   *steps = e->ctrl.steps;
 }
+
+// The FP32 form's stochastically rounded addition (hf2d_tile.h), host build:
+// out[i] = h[i] + dh[i] rounded with key keys[i].
+void emu_add_rounded_stochastically(const float* h, const float* dh, const uint32_t* keys, int64_t n, float* out) {
+  for (int64_t i = 0; i < n; ++i) out[i] = hf::add_rounded_stochastically(h[i], dh[i], keys[i]);
+}
 }

@@ -163,3 +163,18 @@ class EmuSim:
         t, s = C.c_double(), C.c_int64()
         self._L.emu_time(self._h, C.byref(t), C.byref(s))
         return t.value, s.value
+
+
+def add_rounded_stochastically(h, dh, keys):
+    """The FP32 form's stochastically rounded h + dh (host build of the
+    kernels' routine), elementwise over float32 arrays and uint32 keys."""
+    L = lib()
+    if not hasattr(L, "_sr_ready"):
+        L.emu_add_rounded_stochastically.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]
+        L._sr_ready = True
+    h = np.ascontiguousarray(h, np.float32)
+    dh = np.ascontiguousarray(dh, np.float32)
+    keys = np.ascontiguousarray(keys, np.uint32)
+    out = np.empty_like(h)
+    L.emu_add_rounded_stochastically(h.ctypes.data, dh.ctypes.data, keys.ctypes.data, h.size, out.ctypes.data)
+    return out

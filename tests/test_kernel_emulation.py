@@ -187,3 +187,37 @@ def test_active_tile_list_run_until_split_across_calls(tile):
     assert chunked.time() == exact.time() == o.time()
     assert np.array_equal(chunked.get_state(), o.get_state())
     assert np.array_equal(exact.get_state(), o.get_state())
+
+
+def test_fp32_stochastic_rounding_exact_in_expectation():
+    """The FP32 form's rounding of its depth additions (hf2d_tile.h,
+    add_rounded_stochastically), host build: every result is one of the two
+    FP32 neighbours of the exact sum, exact sums stay exact, and over many
+    keys the mean error vanishes where round-to-nearest has a fixed bias
+    (rain of 9.5e-7 m on cells 2-4 m deep: 3.99 ulps, rounded up to 4 on
+    every cell)."""
+    from emu.pyemu import add_rounded_stochastically as add_sr
+
+    rng = np.random.default_rng(11)
+    n = 200_000
+    h = rng.uniform(2.0, 4.0, n).astype(np.float32)
+    dh = np.full(n, 9.5e-7, np.float32)
+    keys = np.arange(n, dtype=np.uint32) * np.uint32(2654435761)
+    out = add_sr(h, dh, keys)
+    exact = h.astype(np.float64) + dh.astype(np.float64)
+    rn = h + dh  # float32 round to nearest
+    other = np.nextafter(rn, np.where(exact > rn, np.float32(np.inf), np.float32(-np.inf)))
+    assert np.all((out == rn) | ((out == other) & (exact != rn.astype(np.float64))))
+    err_rn = float(np.mean(rn.astype(np.float64) - exact))
+    err_sr = float(np.mean(out.astype(np.float64) - exact))
+    assert abs(err_rn) > 1e-9  # the bias being removed: about 3.7e-9 m per addition
+    assert abs(err_sr) < 3e-10, (err_sr, err_rn)
+    # sums that are exact in FP32 stay exact; subtractions (depth updates) too
+    ex_h = rng.uniform(0.25, 0.5, 1000).astype(np.float32)
+    ex_dh = np.float32(2.0 ** -20) * rng.integers(-64, 64, 1000).astype(np.float32)
+    assert np.array_equal(add_sr(ex_h, ex_dh, np.arange(1000, dtype=np.uint32)), ex_h + ex_dh)
+    neg = rng.uniform(0.5, 1.0, n).astype(np.float32)
+    dneg = -rng.uniform(1e-8, 1e-6, n).astype(np.float32)
+    out = add_sr(neg, dneg, keys)
+    exact = neg.astype(np.float64) + dneg.astype(np.float64)
+    assert abs(float(np.mean(out.astype(np.float64) - exact))) < 3e-10
