@@ -238,14 +238,22 @@ def cpu_baseline():
 
 
 def _cpu_model():
-    """The host CPU model as lscpu names it (else /proc/cpuinfo)."""
+    """The host CPU model as lscpu names it, with its family / model /
+    stepping numbers and clock where lscpu gives them (a virtualised host
+    may name only the product line); else /proc/cpuinfo's model name."""
     import subprocess
 
     try:
         out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        f = {}
         for line in out.splitlines():
-            if line.startswith("Model name:"):
-                return line.split(":", 1)[1].strip()
+            if ":" in line:
+                k, v = line.split(":", 1)
+                f[k.strip()] = v.strip()
+        if "Model name" in f:
+            extra = [f"{k.lower()} {f[k]}" for k in ("CPU family", "Model", "Stepping", "CPU max MHz", "BogoMIPS")
+                     if f.get(k)]
+            return f["Model name"] + (f" ({', '.join(extra)})" if extra else "")
     except (OSError, subprocess.SubprocessError):
         pass
     try:
