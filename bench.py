@@ -672,7 +672,69 @@ def batch_bench(args):
     out["f32_over_serial_f64"] = out["f32"]["value"] / out["serial_f64"]
     out["f64_over_serial_f64"] = out["f64"]["value"] / out["serial_f64"]
     log(f"[bench] serial f64: {out['serial_f64'] / 1e9:.2f} G cell-updates/s")
+    out["f32_accuracy"] = batch_accuracy(mix)
     return out, mix
+
+
+def batch_accuracy(mix, n_pick=16, t_end=120.0):
+    """Depth accuracy of the FP32 batch against the FP64 batch on the mix:
+    n_pick scenarios spread over its sizes (sorted by cell count), each run
+    to t_end simulated seconds in an FP64 batch, an FP32 batch and an FP64
+    batch on beds perturbed by up to 1e-9 m (each scenario's own
+    conditioning); per-scenario depth metrics against the FP64 run, reported
+    as the worst and the median over the picked scenarios."""
+    from hydroflux_amd.batch import Batch, ScenarioSpec
+    from hydroflux_amd.metrics import depth_metrics
+
+    order = np.argsort([s.n_cells for s in mix])
+    pick = [mix[int(order[int(round(k * (len(mix) - 1) / (n_pick - 1)))])] for k in range(n_pick)]
+    rng = np.random.default_rng(0xACC)
+    perts = [1e-9 * rng.uniform(-1.0, 1.0, s.bed.shape) for s in pick]
+
+    def run(kind, perturbed):
+        b = Batch(kind, capacity_bytes=16 << 30, max_slots=64)
+        slots = [b.add(ScenarioSpec(bed=s.bed + (p if perturbed else 0.0), manning=s.manning, dx=s.dx,
+                                    bc_kinds=s.bc_kinds, rain_rate=s.rain_rate, sources=s.sources, t_end=t_end,
+                                    max_dt=s.max_dt, cfl=s.cfl)) for s, p in zip(pick, perts)]
+        while True:
+            b.step(64)
+            b.synchronize()
+            t, steps, status, done, used = b.poll()
+            if all(done[sl] or status[sl] for sl in slots):
+                break
+        res = []
+        for s, sl in zip(pick, slots):
+            assert status[sl] == 0 and t[sl] == t_end, (kind, s.n_rows, s.n_cols, status[sl], t[sl])
+            res.append(b.download(sl, (s.n_rows, s.n_cols))[..., 0].copy())
+        b.close()
+        return res
+
+    t0 = time.perf_counter()
+    h64, h32, hp = run("f64", False), run("f32", False), run("f64", True)
+    rows = []
+    for s, a, b32, bp in zip(pick, h64, h32, hp):
+        m32 = depth_metrics(b32, a, s.dx, s.dx)
+        mp = depth_metrics(bp, a, s.dx, s.dx)
+        rows.append({"grid": [s.n_rows, s.n_cols], "fp32_Linf_m": m32["depth_Linf_m"], "fp32_L1_m": m32["depth_L1_m"],
+                     "fp32_iou": m32["extent_iou_1cm"], "fp32_mass": m32["mass_balance_rel_err"],
+                     "fp64_perturbed_Linf_m": mp["depth_Linf_m"], "fp64_perturbed_L1_m": mp["depth_L1_m"],
+                     "fp64_perturbed_mass": mp["mass_balance_rel_err"]})
+    li = np.array([r["fp32_Linf_m"] for r in rows])
+    lp = np.array([r["fp64_perturbed_Linf_m"] for r in rows])
+    out = {"scenarios": n_pick, "t_end_s": t_end,
+           "fp32_Linf_m": {"max": float(li.max()), "median": float(np.median(li))},
+           "fp64_bed_perturbed_1e-9m_Linf_m": {"max": float(lp.max()), "median": float(np.median(lp))},
+           "fp32_within_1mm": int(np.sum(li <= 1e-3)),
+           "fp32_L1_m_max": float(max(r["fp32_L1_m"] for r in rows)),
+           "fp32_iou_min": float(min(r["fp32_iou"] for r in rows)),
+           "fp32_mass_rel_err_max": float(max(abs(r["fp32_mass"]) for r in rows)),
+           "fp64_perturbed_L1_m_max": float(max(r["fp64_perturbed_L1_m"] for r in rows)),
+           "fp64_perturbed_mass_rel_err_max": float(max(abs(r["fp64_perturbed_mass"]) for r in rows)),
+           "per_scenario": rows, "wall_s": time.perf_counter() - t0}
+    log(f"[bench] batch FP32 vs FP64 on {n_pick} mix scenarios to {t_end:.0f} s: Linf max {li.max():.3g} m "
+        f"(median {np.median(li):.3g}); perturbed FP64 Linf max {lp.max():.3g} m (median {np.median(lp):.3g}); "
+        f"{out['fp32_within_1mm']} of {n_pick} within 1 mm")
+    return out
 
 
 def serving_bench(args, mix, f32_tile, batch_f32_value):
