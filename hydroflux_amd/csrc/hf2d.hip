@@ -29,6 +29,7 @@ struct hf2d_sim {
   double dx = 1.0, dy = 1.0;
   int kind = HF2D_F64;
   int n_tan = 0;  // tangents per value: Dual 1, DualN<N> N, else 0
+  uint32_t host_seq = 0;  // host-clocked launches so far (StageCommon::host_seq)
   hipStream_t stream = nullptr;
   void* st[2][3] = {{nullptr}};   // state buffers (double, or float for HF2D_F32)
   void* tg[2][3] = {{nullptr}};   // tangents (Dual; DualN: N per cell, [cell][k])
@@ -424,12 +425,14 @@ template <class T> int host_dt_step(hf2d_sim* s, double dt, bool rk2) {
   StageArgs<T> a = make_args<T>(s, cur, oth, cur);
   a.dt_mode = 0;
   a.dt_host = dt;
+  a.host_seq = ++s->host_seq;
   int rc = launch_stage<T>(s, a, hf::MODE_STAGE, -1);
   if (rc) return rc;
   if (rk2) {
     StageArgs<T> b = make_args<T>(s, oth, cur, cur);
     b.dt_mode = 0;
     b.dt_host = dt;
+    b.host_seq = a.host_seq;
     b.friction = 0;
     b.rain_rate = 0.0;
     b.n_src = 0;
@@ -698,6 +701,7 @@ template <class T> int forcing_only(hf2d_sim* s, double dt, int friction, double
   StageArgs<T> a = make_args<T>(s, s->cur, s->cur, s->cur);
   a.dt_mode = 0;
   a.dt_host = dt;
+  a.host_seq = ++s->host_seq;
   a.friction = friction;
   a.friction_dry_tol = dry_tol;
   a.rain_rate = rain;

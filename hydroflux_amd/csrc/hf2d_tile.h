@@ -191,6 +191,9 @@ struct StageCommon {
   int dt_mode;  // 0: dt_host; 1: from the device clock
   int first_stage;
   double dt_host;
+  // Host-clocked launches (dt_mode 0): their sequence number, the FP32
+  // form's rounding key in place of the device clock's step count.
+  uint32_t host_seq;
   Ctrl* ctrl;
   double cfl, max_dt;
   int cfl_with_bcs;
@@ -1271,7 +1274,14 @@ template <class T, int TW, int TH> struct Stage {
   // rows [gi0, gi0 + th) x columns [gj0, gj0 + tw). Read once per thread,
   // before the launch stores anything, so the per-cell tail reloads none
   // of it.
-  // rain_key: the FP32 form's rounding key of this step (the step count).
+  // The FP32 form's rounding sequence number of a launch: the device
+  // clock's step count, or the host-clocked launch's own number.
+  static HF_HD uint32_t rounding_seq(const StageArgs<T>& a) {
+    if (a.dt_mode == 0) return a.host_seq;
+    return a.ctrl ? (uint32_t)a.ctrl->steps : 0u;
+  }
+
+  // rain_key: the FP32 form's rounding key of this step.
   struct ForcingNow {
     double rain;
     double q;
@@ -1290,7 +1300,7 @@ template <class T, int TW, int TH> struct Stage {
       }
     }
     if constexpr (std::is_same<T, float>::value) {
-      if (f.rain != 0.0 && a.ctrl) f.rain_key = (uint32_t)a.ctrl->steps * 0x9E3779B9u;
+      if (f.rain != 0.0) f.rain_key = rounding_seq(a) * 0x9E3779B9u;
     }
     for (int k = 0; k < a.n_src; ++k) {
       const int64_t r = a.srcs[k].row, cc = a.srcs[k].col;
@@ -1388,7 +1398,7 @@ template <class T, int TW, int TH> struct Stage {
     // same reason as the rain: in slowly varying flow a cell's small
     // increment rounds the same way step after step.
     uint32_t sr_step = 0u;
-    if constexpr (std::is_same<T, float>::value) sr_step = a.ctrl ? (uint32_t)a.ctrl->steps * 0x9E3779B9u : 0u;
+    if constexpr (std::is_same<T, float>::value) sr_step = rounding_seq(a) * 0x9E3779B9u;
     for (int k = tid; k < TW * TH; k += nt) {
       const int li = k / TW, lj = k % TW;
       const int gi = gi0 + li, gj = gj0 + lj;
