@@ -50,12 +50,14 @@ def test_f32_form_tracks_f64_on_the_flood_workload(n):
     sys.stdout.flush()
     assert abs(t32 - t) <= 1e-9 * t
     assert m["wet_cells_ref"] > 1000
-    # measured on MI355X: L1 6.3e-5 / 1.2e-4 m, peak 0.005 / 0.015 m,
-    # IoU 0.925 / 0.952, volume 1.0e-5 / 2.3e-5 (n = 384 / 200)
-    assert m["depth_L1_m"] <= 5e-4
-    assert m["peak_depth_err_m"] <= 5e-2
-    assert m["extent_iou_1cm"] >= 0.85
-    assert m["mass_balance_rel_err"] <= 1e-4
+    # measured on MI355X: L1 5.5e-5 / 8.2e-5 m, peak 1.7e-3 / 1.5e-2 m,
+    # IoU 0.935 / 0.957, volume 8.9e-6 / 1.4e-5 (n = 384 / 200). The
+    # depth L-infinity of this chaotic workload is bounded against the
+    # perturbed-bed FP64 run in test_gpu_configs.py.
+    assert m["depth_L1_m"] <= 2e-4
+    assert m["peak_depth_err_m"] <= 3e-2
+    assert m["extent_iou_1cm"] >= 0.9
+    assert m["mass_balance_rel_err"] <= 5e-5
 
 
 UKEA = os.path.join(os.path.dirname(__file__), "golden", "uk_ea")
