@@ -769,7 +769,15 @@ def serving_bench(args, mix, f32_tile, batch_f32_value):
     def serve(reqs, label):
         srv = Server({"f32": cap, "dual": 6 << 30}, max_slots=512, chunk_steps=args.serve_chunk, keep_results=False,
                      tile_config={"f32": f32_tile})
-        rep = srv.run(reqs)
+        seen = {"n": 0, "t": time.perf_counter()}
+
+        def progress(r):  # a line at most every 20 s: long runs at full durations stay visible
+            seen["n"] += 1
+            if time.perf_counter() - seen["t"] >= 20.0:
+                seen["t"] = time.perf_counter()
+                log(f"[bench]   serving {label}: {seen['n']} of {len(reqs)} requests done")
+
+        rep = srv.run(reqs, on_done=progress)
         srv.close()
         s = rep.summary()
         log(f"[bench] serving {label}: {len(reqs)} requests, p50 {s['latency_p50_s']:.2f} s, "
